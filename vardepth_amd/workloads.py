@@ -1,0 +1,343 @@
+"""Region batches (the engine's input layout) and seeded synthetic workloads.
+
+A :class:`RegionBatch` is the CSR form of many active regions, each with its
+own reads, base qualities and candidate haplotypes -- the same data that the
+reference's region call ``pair_scores_many(lib, sequences, qualities, haps)``
+(caller/native/caller/base/caller.py:151-160) receives one region at a time.
+Output for region ``k`` is the row-major ``R_k x H_k`` natural-log likelihood
+block at ``out_off[k]``.
+
+The generators restate the benchmark workloads as concrete seeded inputs
+(numpy PCG64):
+
+* :func:`single_region` -- one active region, 64 reads x 8 haplotypes of
+  200-400 bp, 150 bp reads, Phred U{10..40}, gap open Q45, continuation Q10.
+* :func:`uneven_regions` -- 4096 regions, 2-32 haplotypes of 150-600 bp,
+  low/high-depth read counts, lognormal read lengths clipped to 50-250 bp,
+  bimodal Phred (80% in 30-40, 20% in 2-20).
+* :func:`precision_stress` -- 1024 regions of 250 bp reads against 1-2 kb
+  indel-dense haplotypes at Phred 2-15.
+* :func:`hmm_sampled_regions` -- reads sampled from the PairHMM itself (so
+  gap parameters are identifiable), with the true haplotype of every read.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
+GAP_Q45 = 10.0 ** -4.5      # "indel quality Q45 open"
+EXT_Q10 = 10.0 ** -1.0      # "Q10 continuation"
+
+
+@dataclasses.dataclass
+class RegionBatch:
+    """CSR batch of active regions (all offsets int64, all bases uint8)."""
+
+    read_bytes: np.ndarray
+    qual_bytes: np.ndarray
+    read_off: np.ndarray
+    hap_bytes: np.ndarray
+    hap_off: np.ndarray
+    region_reads: np.ndarray
+    region_haps: np.ndarray
+    gap: float = 0.001
+    ext: float = 0.1
+    read_true_hap: np.ndarray | None = None   # global hap index per read (or None)
+
+    def __post_init__(self):
+        self.read_bytes = np.ascontiguousarray(self.read_bytes, dtype=np.uint8)
+        self.qual_bytes = np.ascontiguousarray(self.qual_bytes, dtype=np.uint8)
+        self.read_off = np.ascontiguousarray(self.read_off, dtype=np.int64)
+        self.hap_bytes = np.ascontiguousarray(self.hap_bytes, dtype=np.uint8)
+        self.hap_off = np.ascontiguousarray(self.hap_off, dtype=np.int64)
+        self.region_reads = np.ascontiguousarray(self.region_reads, dtype=np.int64)
+        self.region_haps = np.ascontiguousarray(self.region_haps, dtype=np.int64)
+        if len(self.read_bytes) != len(self.qual_bytes):
+            raise ValueError("read and quality byte arrays differ in length")
+        if self.read_off[-1] != len(self.read_bytes) or self.hap_off[-1] != len(self.hap_bytes):
+            raise ValueError("offsets do not cover the byte arrays")
+        if len(self.region_reads) != len(self.region_haps):
+            raise ValueError("region read/haplotype ranges differ in count")
+
+    # -- shape helpers -------------------------------------------------
+    @property
+    def nregions(self) -> int:
+        return len(self.region_reads) - 1
+
+    @property
+    def nreads(self) -> int:
+        return len(self.read_off) - 1
+
+    @property
+    def nhaps(self) -> int:
+        return len(self.hap_off) - 1
+
+    def read_lengths(self) -> np.ndarray:
+        return np.diff(self.read_off)
+
+    def hap_lengths(self) -> np.ndarray:
+        return np.diff(self.hap_off)
+
+    def region_counts(self):
+        return np.diff(self.region_reads), np.diff(self.region_haps)
+
+    def out_off(self) -> np.ndarray:
+        r, h = self.region_counts()
+        out = np.zeros(self.nregions + 1, dtype=np.int64)
+        np.cumsum(r * h, out=out[1:])
+        return out
+
+    @property
+    def npairs(self) -> int:
+        return int(self.out_off()[-1])
+
+    def cells(self) -> int:
+        """Sum over pairs of n*m (the reference's dp_cells, accelerator.py:155)."""
+        rl, hl = self.read_lengths(), self.hap_lengths()
+        total = 0
+        for k in range(self.nregions):
+            a = int(rl[self.region_reads[k]:self.region_reads[k + 1]].sum())
+            b = int(hl[self.region_haps[k]:self.region_haps[k + 1]].sum())
+            total += a * b
+        return total
+
+    def region(self, k):
+        """(reads, quals, haps) of region k as Python str/bytes lists."""
+        r0, r1 = self.region_reads[k], self.region_reads[k + 1]
+        h0, h1 = self.region_haps[k], self.region_haps[k + 1]
+        reads = [self.read_bytes[self.read_off[i]:self.read_off[i + 1]].tobytes().decode("latin-1")
+                 for i in range(r0, r1)]
+        quals = [self.qual_bytes[self.read_off[i]:self.read_off[i + 1]].tobytes() for i in range(r0, r1)]
+        haps = [self.hap_bytes[self.hap_off[i]:self.hap_off[i + 1]].tobytes().decode("latin-1")
+                for i in range(h0, h1)]
+        return reads, quals, haps
+
+    def subset(self, regions) -> "RegionBatch":
+        """A new batch holding the given regions (in the given order)."""
+        return from_regions([self.region(k) for k in regions], gap=self.gap, ext=self.ext)
+
+
+def from_regions(regions, gap=0.001, ext=0.1) -> RegionBatch:
+    """Build a batch from a list of (reads, quals, haps) Python lists."""
+    rb, qb, ro, hb, ho, rr, rh = [], [], [0], [], [0], [0], [0]
+    for reads, quals, haps in regions:
+        if len(reads) != len(quals):
+            raise ValueError("Sequence/quality batch lengths differ")
+        for s, q in zip(reads, quals):
+            s = s.encode("latin-1") if isinstance(s, str) else bytes(s)
+            q = bytes(q) if isinstance(q, (bytes, bytearray)) else np.asarray(q, dtype=np.uint8).tobytes()
+            if len(s) != len(q):
+                raise ValueError("Read and quality lengths differ")
+            rb.append(s)
+            qb.append(q)
+            ro.append(ro[-1] + len(s))
+        for h in haps:
+            h = h.encode("latin-1") if isinstance(h, str) else bytes(h)
+            hb.append(h)
+            ho.append(ho[-1] + len(h))
+        rr.append(rr[-1] + len(reads))
+        rh.append(rh[-1] + len(haps))
+    return RegionBatch(
+        read_bytes=np.frombuffer(b"".join(rb), dtype=np.uint8).copy(),
+        qual_bytes=np.frombuffer(b"".join(qb), dtype=np.uint8).copy(),
+        read_off=np.array(ro, dtype=np.int64),
+        hap_bytes=np.frombuffer(b"".join(hb), dtype=np.uint8).copy(),
+        hap_off=np.array(ho, dtype=np.int64),
+        region_reads=np.array(rr, dtype=np.int64),
+        region_haps=np.array(rh, dtype=np.int64),
+        gap=gap, ext=ext)
+
+
+# ---------------------------------------------------------------------------
+# generators
+
+def _mutate(rng, base: np.ndarray, snvs: int, indels: int, max_indel: int = 10) -> np.ndarray:
+    """Copy of ``base`` with ``snvs`` substitutions and ``indels`` indels of 1..max_indel bp."""
+    h = base.copy()
+    for _ in range(snvs):
+        p = int(rng.integers(0, len(h)))
+        h[p] = BASES[(np.searchsorted(BASES, h[p]) + int(rng.integers(1, 4))) % 4]
+    for _ in range(indels):
+        size = int(rng.integers(1, max_indel + 1))
+        p = int(rng.integers(1, max(2, len(h) - size)))
+        if rng.random() < 0.5:
+            h = np.concatenate([h[:p], BASES[rng.integers(0, 4, size)], h[p:]])
+        else:
+            h = np.concatenate([h[:p], h[p + size:]])
+    return h
+
+
+def _substitute(rng, seq: np.ndarray, qual: np.ndarray) -> np.ndarray:
+    """Substitute each base with probability 10^(-q/10) to one of the other three."""
+    err = np.power(10.0, -qual.astype(np.float64) / 10.0)
+    hit = rng.random(len(seq)) < err
+    if hit.any():
+        code = np.searchsorted(BASES, seq[hit])
+        seq = seq.copy()
+        seq[hit] = BASES[(code + rng.integers(1, 4, int(hit.sum()))) % 4]
+    return seq
+
+
+class _Builder:
+    def __init__(self):
+        self.reads, self.quals, self.haps = [], [], []
+        self.rcount, self.hcount, self.truth = [], [], []
+
+    def add(self, haps, reads, quals, truth=None):
+        base = sum(self.hcount)
+        self.haps.extend(haps)
+        self.reads.extend(reads)
+        self.quals.extend(quals)
+        self.rcount.append(len(reads))
+        self.hcount.append(len(haps))
+        if truth is not None:
+            self.truth.extend(int(t) + base for t in truth)
+
+    def batch(self, gap, ext) -> RegionBatch:
+        def cat(arrs):
+            if not arrs:
+                return np.zeros(0, np.uint8), np.zeros(1, np.int64)
+            off = np.zeros(len(arrs) + 1, dtype=np.int64)
+            np.cumsum([len(a) for a in arrs], out=off[1:])
+            return np.concatenate(arrs).astype(np.uint8), off
+        rb, ro = cat(self.reads)
+        qb, _ = cat(self.quals)
+        hb, ho = cat(self.haps)
+        rr = np.zeros(len(self.rcount) + 1, dtype=np.int64)
+        rh = np.zeros(len(self.hcount) + 1, dtype=np.int64)
+        np.cumsum(self.rcount, out=rr[1:])
+        np.cumsum(self.hcount, out=rh[1:])
+        truth = np.array(self.truth, dtype=np.int64) if self.truth else None
+        return RegionBatch(rb, qb, ro, hb, ho, rr, rh, gap=gap, ext=ext, read_true_hap=truth)
+
+
+def _region_haps(rng, length, count, snv_max=5, indel_max=2, max_indel=10):
+    base = BASES[rng.integers(0, 4, length)]
+    return [_mutate(rng, base, int(rng.integers(0, snv_max + 1)), int(rng.integers(0, indel_max + 1)),
+                    max_indel) for _ in range(count)]
+
+
+def _sample_reads(rng, haps, lengths, qual_fn):
+    reads, quals, truth = [], [], []
+    for n in lengths:
+        h = int(rng.integers(0, len(haps)))
+        hap = haps[h]
+        n = int(min(n, len(hap)))
+        o = int(rng.integers(0, len(hap) - n + 1))
+        q = qual_fn(n)
+        reads.append(_substitute(rng, hap[o:o + n], q))
+        quals.append(q)
+        truth.append(h)
+    return reads, quals, truth
+
+
+def single_region(seed: int = 41001, reads: int = 64, haps: int = 8, read_len: int = 150,
+                  gap: float = GAP_Q45, ext: float = EXT_Q10) -> RegionBatch:
+    """One active region: 64 reads x 8 haplotypes of 200-400 bp."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = _Builder()
+    length = int(rng.integers(200, 401))
+    hs = _region_haps(rng, length, haps)
+    rs, qs, truth = _sample_reads(rng, hs, [read_len] * reads,
+                                  lambda n: rng.integers(10, 41, n).astype(np.uint8))
+    b.add(hs, rs, qs, truth)
+    return b.batch(gap, ext)
+
+
+def _bimodal_quals(rng):
+    def fn(n):
+        hi = rng.random(n) < 0.8
+        q = np.where(hi, rng.integers(30, 41, n), rng.integers(2, 21, n))
+        return q.astype(np.uint8)
+    return fn
+
+
+def uneven_regions(seed: int = 41002, regions: int = 4096, gap: float = 0.001,
+                   ext: float = 0.1) -> RegionBatch:
+    """4096 uneven active regions (2-32 haplotypes of 150-600 bp; 2-30 or 200-2000 reads)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = _Builder()
+    qfn = _bimodal_quals(rng)
+    for _ in range(regions):
+        h = int(rng.integers(2, 33))
+        length = int(rng.integers(150, 601))
+        hs = _region_haps(rng, length, h)
+        r = int(rng.integers(2, 31)) if rng.random() < 0.85 else int(rng.integers(200, 2001))
+        lens = np.clip(np.rint(rng.lognormal(math.log(150.0), 0.35, r)), 50, 250).astype(int)
+        rs, qs, truth = _sample_reads(rng, hs, lens, qfn)
+        b.add(hs, rs, qs, truth)
+    return b.batch(gap, ext)
+
+
+def precision_stress(seed: int = 41003, regions: int = 1024, reads: int = 16,
+                     gap: float = GAP_Q45, ext: float = EXT_Q10) -> RegionBatch:
+    """250 bp low-quality reads against 1-2 kb haplotypes with one indel per 50 bp."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = _Builder()
+    for _ in range(regions):
+        h = int(rng.integers(2, 5))
+        length = int(rng.integers(1000, 2001))
+        base = BASES[rng.integers(0, 4, length)]
+        hs = [_mutate(rng, base, int(rng.integers(0, 6)), max(1, length // 50), 10) for _ in range(h)]
+        rs, qs, truth = _sample_reads(rng, hs, [250] * reads,
+                                      lambda n: rng.integers(2, 16, n).astype(np.uint8))
+        b.add(hs, rs, qs, truth)
+    return b.batch(gap, ext)
+
+
+def _hmm_read(rng, hap: np.ndarray, n: int, qual: np.ndarray, gap: float, ext: float):
+    """Sample a read of n bases from the PairHMM: start uniformly in the haplotype,
+    M->I / M->D with probability gap, I->I / D->D with probability ext, match emission
+    1-e else a uniform other base, insertions uniform over ACGT."""
+    out = np.empty(n, dtype=np.uint8)
+    j = int(rng.integers(0, max(1, len(hap) - n + 1)))
+    state = 0  # 0=M, 1=I, 2=D
+    i = 0
+    err = np.power(10.0, -qual.astype(np.float64) / 10.0)
+    while i < n:
+        u = rng.random()
+        if state == 0:
+            state = 1 if u < gap else (2 if u < 2 * gap else 0)
+        else:
+            state = state if u < ext else 0
+        if state == 2:
+            j += 1
+            if j >= len(hap):
+                state = 0
+            continue
+        if state == 1:
+            out[i] = BASES[rng.integers(0, 4)]
+            i += 1
+            continue
+        if j >= len(hap):
+            j = len(hap) - 1
+        base = hap[j]
+        if rng.random() < err[i]:
+            base = BASES[(np.searchsorted(BASES, base) + int(rng.integers(1, 4))) % 4]
+        out[i] = base
+        i += 1
+        j += 1
+    return out
+
+
+def hmm_sampled_regions(seed: int = 41004, regions: int = 1024, reads: int = 100, haps: int = 20,
+                        read_len: int = 150, gap: float = GAP_Q45, ext: float = EXT_Q10,
+                        sample_gap: float = 0.01, sample_ext: float = 0.3) -> RegionBatch:
+    """Regions whose reads are drawn from the PairHMM itself; truth hap per read recorded."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = _Builder()
+    for _ in range(regions):
+        length = int(rng.integers(200, 401))
+        hs = _region_haps(rng, length, haps)
+        rs, qs, truth = [], [], []
+        for _ in range(reads):
+            h = int(rng.integers(0, haps))
+            q = rng.integers(10, 41, read_len).astype(np.uint8)
+            rs.append(_hmm_read(rng, hs[h], read_len, q, sample_gap, sample_ext))
+            qs.append(q)
+            truth.append(h)
+        b.add(hs, rs, qs, truth)
+    return b.batch(gap, ext)
