@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""PairHMM GCUPS on one MI355X: batched uneven active regions in one persistent launch.
+
+A step is one forward pass of the whole batch (planning, FP32 anti-diagonal
+kernel, FP64 rescue) over inputs already resident in HBM.  The default
+workload is 4096 uneven regions (2-32 haplotypes of 150-600 bp, 2-30 or
+200-2000 reads of 50-250 bp, bimodal Phred), seed 41002.
+
+The JSON line carries:
+* value      -- whole-job GCUPS = sum(n*m) * steps / wall seconds
+* roofline   -- the FP32 kernel's algorithmic rate (11 flops per cell) over its
+                own event-timed duration, against MI355X's FP32 vector peak
+* cpu_baseline -- the reference's CPU path (trie_many + pairhmm_pairs
+                fallback, compiled from the reference sources into oracle/_ref)
+                on a time-bounded sample of the same workload, on host threads
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--regions N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "PairHMM GCUPS (fwd, fwd+bwd) at <=1e-4 log10 err vs FP64; % of FP32 peak"
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_CELL = 11
+
+
+def available_cpus() -> int:
+    """Affinity intersected with the cgroup CPU quota (cpu_schedule.py:5-18)."""
+    count = len(os.sched_getaffinity(0))
+    try:
+        for line in Path("/proc/self/cgroup").read_text().splitlines():
+            if line.startswith("0::"):
+                path = Path("/sys/fs/cgroup") / line.split(":", 2)[2].lstrip("/")
+                for d in [path, *path.parents]:
+                    f = d / "cpu.max"
+                    if f.exists():
+                        quota, period = f.read_text().split()
+                        if quota != "max":
+                            count = min(count, max(1, math.floor(int(quota) / int(period))))
+                        break
+    except (OSError, ValueError):
+        pass
+    return max(1, count)
+
+
+def workload(name: str, regions: int):
+    from vardepth_amd import workloads
+    if name == "uneven":
+        return workloads.uneven_regions(seed=41002, regions=regions), "uneven-regions (4096 active regions, seed 41002)"
+    if name == "single":
+        return workloads.single_region(seed=41001), "single-region 64x8 (seed 41001)"
+    if name == "stress":
+        return workloads.precision_stress(seed=41003, regions=regions), "precision-stress (seed 41003)"
+    raise ValueError(name)
+
+
+def fp32_cells(batch, mode: int) -> int:
+    """Cells of the pairs the FP32 kernel processes (reads of <= 256 rows)."""
+    rl = batch.read_lengths() + (1 if mode == 1 else 0)
+    hl = batch.hap_lengths()
+    total = 0
+    for k in range(batch.nregions):
+        r = rl[batch.region_reads[k]:batch.region_reads[k + 1]]
+        r = r[(r > 0) & (r <= 256)] - (1 if mode == 1 else 0)
+        total += int(r.sum()) * int(hl[batch.region_haps[k]:batch.region_haps[k + 1]].sum())
+    return total
+
+
+def cpu_baseline(batch, mode: int, seconds: float, threads: int):
+    """Reference CPU chain on a time-bounded sample of regions (largest-first)."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    ref = oracle.ref()
+    kind = "reference" if ref is not None else "port"
+    rng = np.random.default_rng(7)
+    order = rng.permutation(batch.nregions)
+    rl, hl = batch.read_lengths(), batch.hap_lengths()
+
+    def one(k):
+        reads, quals, haps = batch.region(k)
+        if ref is not None:
+            ri = sorted(range(len(reads)), key=lambda i: len(reads[i]))   # trie_vector.py:30
+            hs = sorted(haps)                                            # trie_vector.py:27
+            out, _ = ref.trie_many([reads[i] for i in ri], [quals[i] for i in ri], hs, mode)
+            nan = np.argwhere(np.isnan(out))
+            if len(nan):
+                jobs = [(reads[ri[a]], quals[ri[a]], hs[b]) for a, b in nan]
+                out[nan[:, 0], nan[:, 1]] = ref.pairhmm_pairs(jobs, mode)   # trie_vector.py:40-41
+        else:
+            sub = batch.subset([k])
+            oracle.regions_cpu(sub, mode=mode, threads=1)
+        a = int(rl[batch.region_reads[k]:batch.region_reads[k + 1]].sum())
+        b = int(hl[batch.region_haps[k]:batch.region_haps[k + 1]].sum())
+        return a * b
+
+    def run(nthreads, budget):
+        cells = 0
+        done = 0
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=nthreads) as ex:
+            futs = []
+            it = iter(order)
+            for _ in range(nthreads * 2):
+                k = next(it, None)
+                if k is not None:
+                    futs.append(ex.submit(one, int(k)))
+            while futs:
+                f = futs.pop(0)
+                cells += f.result()
+                done += 1
+                if time.perf_counter() - t0 < budget:
+                    k = next(it, None)
+                    if k is not None:
+                        futs.append(ex.submit(one, int(k)))
+        return cells, done, time.perf_counter() - t0
+
+    cells, done, dt = run(threads, seconds)
+    c1, d1, t1 = run(1, max(2.0, seconds / 5))
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
+            "sample": "%d of %d regions of the same workload (%.3g cells, %.1f s wall), FP64 %s; "
+                      "1 core: %.3f GCUPS over %d regions" % (
+                          done, batch.nregions, cells, dt,
+                          "trie_many+pairhmm_pairs compiled from the reference sources" if kind == "reference"
+                          else "C restatement of trie_many+pairhmm_pairs", c1 / t1 / 1e9, d1),
+            "gcups_1core": c1 / t1 / 1e9, "cpu_model": model}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--regions", type=int, default=4096)
+    ap.add_argument("--workload", default="uneven", choices=["uneven", "single", "stress"])
+    ap.add_argument("--mode", type=int, default=0, choices=[0, 1, 2])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "float64"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the regions shard across GPUs with no exchange, "
+              "so only --gpus 1 is run here (asked for %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from vardepth_amd import engine
+
+    t0 = time.perf_counter()
+    batch, wname = workload(args.workload, args.regions)
+    gen_s = time.perf_counter() - t0
+    cells = batch.cells()
+    cells32 = fp32_cells(batch, args.mode)
+    print("workload %s: %d regions %d pairs %.4g cells (generated in %.1fs)" % (
+        wname, batch.nregions, batch.npairs, cells, gen_s), file=sys.stderr, flush=True)
+
+    dev = engine.DeviceBatch(batch)
+    for _ in range(args.warmup):
+        dev.forward(mode=args.mode, precision=args.precision, stats=True)
+    torch.cuda.synchronize()
+    steps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        steps.append(dev.forward(mode=args.mode, precision=args.precision, stats=True))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    gcups = cells * args.steps / wall / 1e9
+    fwd32 = float(np.mean([s["fwd32_ms"] for s in steps]))
+    achieved = cells32 * FLOPS_PER_CELL / (fwd32 * 1e-3) / 1e12 if fwd32 > 0 else 0.0
+    last = steps[-1]
+    print("step %.2f ms: plan %.2f fwd32 %.2f fwd64 %.2f ms; flagged %d, fp64 pairs %d" % (
+        ms_step, last["plan_ms"], last["fwd32_ms"], last["fwd64_ms"], last["flagged"], last["fp64_pairs"]),
+        file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(available_cpus(), 16)
+        cpu = cpu_baseline(batch, args.mode, args.cpu_seconds, threads)
+
+    line = {
+        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": wname, "regions": batch.nregions, "pairs": batch.npairs, "cells": cells,
+                   "mode": args.mode, "precision": "fp32 bulk + fp64 rescue" if args.precision == "mixed" else "fp64",
+                   "gap": batch.gap, "ext": batch.ext,
+                   "flagged_pairs": last["flagged"], "fp64_pairs": last["fp64_pairs"],
+                   "stage_ms": {"plan": round(last["plan_ms"], 3), "fwd32": round(last["fwd32_ms"], 3),
+                                "fwd64": round(last["fwd64_ms"], 3)}},
+        "roofline": {"bound": "valu", "kernel": "k_fwd32", "achieved": round(achieved, 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": None, "flops_per_cell": FLOPS_PER_CELL, "cells_per_launch": cells32,
+                     "kernel_ms": round(fwd32, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * vdhmm -- MI355X PairHMM engine, C ABI.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-ins with the exact names, argument order and return conventions of
+ *    the reference's native PairHMM libraries (caller/native/caller), so that
+ *    the reference's ctypes loaders can bind this library instead:
+ *      pairhmm_pairs             batch_hmm.cpp:192   (ctypes: cpu_batch.py:14-17)
+ *      trie_many                 trie_vector.cpp:284 (ctypes: trie_vector.py:11-13)
+ *      read_haplotype            wide.cpp:9          (ctypes: accelerator.py:50-51)
+ *      read_haplotype_batch      wide.cpp:185        (ctypes: whole_pairhmm.py:43-44)
+ *      read_haplotype_batch_mode wide.cpp:349        (ctypes: whole_pairhmm.py:41-42)
+ *      wide_capability           wide.cpp:342        (ctypes: whole_pairhmm.py:45-46)
+ *      anchored_pairhmm          anchored.cpp:5      (ctypes: indel_refine.py:56-57)
+ *    base/core/core.cpp:9,176 exports a second read_haplotype /
+ *    read_haplotype_batch (mode-0 semantics); those are vd_core_read_haplotype
+ *    and vd_core_read_haplotype_batch here, and libvdhmm_core.so re-exports
+ *    them under the reference names.
+ *    All return natural-log likelihoods in FP64: -inf for an empty read or
+ *    haplotype; trie_many returns NaN where the likelihood is <= 1e-200
+ *    (trie_vector.cpp:277), as the reference does.
+ *
+ * 2. The batched region engine (vd_forward*): many active regions in one
+ *    call, CSR layout, explicit gap / continuation / per-quality-bin error
+ *    parameters, FP32 bulk with FP64 rescue or all-FP64.
+ *
+ * The library keeps one HIP context per process and is not re-entrant across
+ * host threads: calls on one process are serialised by an internal mutex.
+ */
+#ifndef VDHMM_H
+#define VDHMM_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-ins ---------------------------------------------------------- */
+
+/* batch_hmm.cpp:192.  Dense per-pair likelihoods, modes 0/1/2.  Returns 1 if
+ * count<0 or mode not in {0,1,2}, 2 on a device error, else 0.  `lanes` is
+ * accepted and ignored (it selects a CPU SIMD width in the reference and does
+ * not change values).  Computed in FP64 on the GPU. */
+int pairhmm_pairs(const char *const *reads, const unsigned char *const *quals, const int *rn,
+                  const char *const *haps, const int *hn, int count, int mode, int lanes,
+                  double *output);
+
+/* trie_vector.cpp:284.  nr reads x nh haplotypes, out row-major nr x nh.
+ * *cells receives the reference's evaluated-cell counter (AVX-512 lane
+ * grouping).  Returns 1 if nr<0, nh<0 or mode not in {0,1,2}.  Precision:
+ * FP32 + FP64 rescue unless VARDEPTH_CUDA_PRECISION=float64. */
+int trie_many(const char *const *reads, const unsigned char *const *quals, const int *rn, int nr,
+              const char *const *haps, const int *hn, int nh, int mode, int lanes, double *out,
+              uint64_t *cells);
+
+/* wide.cpp:9 -- semiglobal (mode 2 model) with explicit gap / ext. FP64. */
+double read_haplotype(const char *read, const unsigned char *qual, int n, const char *hap, int m,
+                      double gap, double ext);
+/* wide.cpp:185 -- one read against count haplotypes. FP64. */
+void read_haplotype_batch(const char *read, const unsigned char *qual, int n, const char *const *hap,
+                          const int *lengths, int count, double gap, double ext, double *out);
+/* wide.cpp:349 -- as above; `mode` is the reference's SIMD lane request
+ * (0/2/4/8) and does not change values. */
+void read_haplotype_batch_mode(const char *read, const unsigned char *qual, int n,
+                               const char *const *haps, const int *lengths, int count, double gap,
+                               double ext, double *out, int mode);
+/* wide.cpp:342 -- widest lane request the batch entry honours (8). */
+int wide_capability(void);
+/* anchored.cpp:5 -- global/anchored (mode 1) with explicit gap / ext. FP64. */
+double anchored_pairhmm(const char *read, const unsigned char *qual, int n, const char *hap, int m,
+                        double gap, double extension);
+/* base/core/core.cpp:9 / :176 -- semiglobal, per-row normalised (mode 0). FP64. */
+double vd_core_read_haplotype(const char *read, const unsigned char *qual, int n, const char *hap,
+                              int m, double gap, double ext);
+void vd_core_read_haplotype_batch(const char *read, const unsigned char *qual, int n,
+                                  const char *const *hap, const int *lengths, int count, double gap,
+                                  double ext, double *out);
+
+/* ---- batched region engine --------------------------------------------- */
+
+enum { VD_MODE_SEMIGLOBAL = 0, VD_MODE_ANCHORED = 1, VD_MODE_SEMIGLOBAL_RESCALE = 2 };
+enum { VD_PREC_MIXED = 0, VD_PREC_FP64 = 1 };
+
+typedef struct vd_params {
+  double gap;      /* gap open: M->I and M->D */
+  double ext;      /* gap continuation: I->I and D->D */
+  double err[64];  /* error probability of quality bin min(q, 63) */
+} vd_params;
+
+/* Region batch.  Read i has bases/quals [read_off[i], read_off[i+1]);
+ * haplotype h has bases [hap_off[h], hap_off[h+1]); region k owns reads
+ * [region_reads[k], region_reads[k+1]) and haplotypes
+ * [region_haps[k], region_haps[k+1]) and writes the row-major R_k x H_k
+ * block at out + out_off[k]. */
+typedef struct vd_batch {
+  const uint8_t *read_bytes;
+  const uint8_t *qual_bytes;
+  const int64_t *read_off;
+  const uint8_t *hap_bytes;
+  const int64_t *hap_off;
+  const int64_t *region_reads;
+  const int64_t *region_haps;
+  const int64_t *out_off;
+  int64_t nregions, nreads, nhaps;
+} vd_batch;
+
+typedef struct vd_stats {
+  uint64_t pairs;        /* read x haplotype pairs in the batch */
+  uint64_t cells;        /* sum of n*m (accelerator.py:155 dp_cells) */
+  uint64_t fp32_units;   /* FP32 work units (4 pairs each) */
+  uint64_t flagged;      /* FP32 pairs sent to the FP64 rescue */
+  uint64_t fp64_pairs;   /* pairs computed in FP64 (rescued + long + forced) */
+  uint64_t fp64_bad;     /* FP64 results that were not finite (should be 0) */
+  uint64_t overflow;     /* FP64 list overflow (must be 0) */
+  double device_ms;      /* GPU time of the batch (events) */
+  double plan_ms;        /* GPU time of the planning kernels */
+  double fwd32_ms;       /* GPU time of the FP32 anti-diagonal kernel */
+  double fwd64_ms;       /* GPU time of the FP64 kernel (rescue / long / forced) */
+  double total_ms;       /* wall time of the call */
+} vd_stats;
+
+/* Reference error table 10^(-min(q,60)/10) for the 64 bins, and gap / ext. */
+void vd_params_default(vd_params *p, double gap, double ext);
+
+/* Host pointers in, host out.  Returns 0, or 2 with vd_last_error() set. */
+int vd_forward(const vd_batch *batch, const vd_params *params, int mode, int precision, double *out,
+               vd_stats *stats);
+
+/* Device pointers in and out (already resident in HBM), launched on `stream`
+ * (a hipStream_t, NULL = default stream).  npairs = out_off[nregions];
+ * max_hap_len bounds every haplotype length.  If stats is non-NULL the call
+ * synchronises the stream and fills it; with stats NULL it only enqueues. */
+int vd_forward_device(const vd_batch *dev_batch, const vd_params *params, int mode, int precision,
+                      int64_t npairs, int max_hap_len, double *dev_out, void *stream,
+                      vd_stats *stats);
+
+/* Message of the last failed call on this process. */
+const char *vd_last_error(void);
+/* Device index used by the library (VARDEPTH_CUDA_DEVICE, default 0). */
+int vd_device(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

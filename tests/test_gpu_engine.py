@@ -1,0 +1,107 @@
+"""HIP engine vs the FP64 oracle and the reference's golden vectors (needs an MI355X).
+
+Tolerances: FP32+FP64-rescue ("mixed") within 1e-4 log10 = 2.3026e-4 in ln
+per pair (the engine's accuracy contract); all-FP64 within 1e-9 ln (the
+reference FP64 CUDA path is held to 2e-9, test_accelerator.py:50)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.golden import fixtures as fx
+from vardepth_amd import engine, workloads, _lib
+
+pytestmark = pytest.mark.gpu
+TOL_MIXED = 1e-4 * math.log(10)
+TOL_FP64 = 1e-9
+G = fx.load()
+
+
+def close(actual, expected, tol):
+    actual, expected = np.asarray(actual), np.asarray(expected)
+    assert actual.shape == expected.shape
+    inf_a, inf_e = np.isneginf(actual), np.isneginf(expected)
+    np.testing.assert_array_equal(inf_a, inf_e)
+    ok = ~inf_e
+    assert np.isfinite(actual[ok]).all(), actual[ok][~np.isfinite(actual[ok])]
+    err = np.abs(actual[ok] - expected[ok])
+    assert err.size == 0 or err.max() <= tol, (err.max(), np.argmax(err))
+    return float(err.max()) if err.size else 0.0
+
+
+def region_batch_from_pairs(pairs, gap=0.001, ext=0.1):
+    """Each pair as its own 1x1 region."""
+    return workloads.from_regions([([s], [q], [h]) for s, q, h in pairs], gap, ext)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("precision,tol", [("mixed", TOL_MIXED), ("float64", TOL_FP64)])
+def test_active_region_vs_oracle(mode, precision, tol):
+    b = workloads.single_region(seed=41001)
+    out, st = engine.forward(b, mode=mode, precision=precision, stats=True)
+    reads, quals, haps = b.region(0)
+    expect = np.array([oracle.dense(s, q, h, mode, b.gap, b.ext) for s, q in zip(reads, quals) for h in haps])
+    err = close(out, expect, tol)
+    print("mode %d %s: max |ln err| %.3g, flagged %d, fp64 %d" % (mode, precision, err, st["flagged"],
+                                                                   st["fp64_pairs"]))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_golden_region_dense(mode):
+    """The 64x8 region at g=.001, x=.1 against the reference's pairhmm_pairs outputs."""
+    pairs = fx.pairs("region")
+    R, H = (int(v) for v in G["region_shape"])
+    b = workloads.from_regions([([pairs[i * H][0] for i in range(R)], [pairs[i * H][1] for i in range(R)],
+                                 [pairs[j][2] for j in range(H)])])
+    close(engine.forward(b, mode=mode, precision="float64"), G["region_dense_m%d" % mode], TOL_FP64)
+    close(engine.forward(b, mode=mode, precision="mixed"), G["region_dense_m%d" % mode], TOL_MIXED)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_grid_shapes(mode):
+    """Reference CUDA-test shapes (1,1)..(700,750), each pair its own region."""
+    pairs = fx.pairs("grid")
+    b = region_batch_from_pairs(pairs)
+    close(engine.forward(b, mode=mode, precision="float64"), G["grid_dense_m%d" % mode], TOL_FP64)
+    close(engine.forward(b, mode=mode, precision="mixed"), G["grid_dense_m%d" % mode], TOL_MIXED)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_edge_cases(mode):
+    pairs = fx.pairs("edge")
+    b = region_batch_from_pairs(pairs)
+    close(engine.forward(b, mode=mode, precision="float64"), G["edge_dense_m%d" % mode], TOL_FP64)
+    close(engine.forward(b, mode=mode, precision="mixed"), G["edge_dense_m%d" % mode], TOL_MIXED)
+
+
+def test_precision_stress_rescue():
+    """Low-quality 250 bp reads vs 1-2 kb indel-dense haplotypes: FP32 bulk, FP64 rescue."""
+    b = workloads.precision_stress(seed=41003, regions=6)
+    out, st = engine.forward(b, mode=0, precision="mixed", stats=True)
+    expect, _ = oracle.regions_cpu(b, mode=0)
+    err = close(out, expect, TOL_MIXED)
+    assert st["flagged"] > 0 and st["fp64_pairs"] >= st["flagged"]
+    print("stress: pairs %d flagged %d max err %.3g" % (st["pairs"], st["flagged"], err))
+
+
+def test_uneven_regions_subset():
+    b = workloads.uneven_regions(seed=41002, regions=24)
+    out, st = engine.forward(b, mode=0, precision="mixed", stats=True)
+    expect, _ = oracle.regions_cpu(b, mode=0)
+    err = close(out, expect, TOL_MIXED)
+    print("uneven: pairs %d cells %d flagged %d max err %.3g ms %.2f" % (
+        st["pairs"], st["cells"], st["flagged"], err, st["device_ms"]))
+
+
+def test_output_indexing_is_exact():
+    """Every output slot is written by exactly its own pair: permute regions and compare."""
+    b = workloads.uneven_regions(seed=7, regions=12)
+    out = engine.forward(b, mode=0, precision="float64")
+    order = list(range(b.nregions))[::-1]
+    b2 = b.subset(order)
+    out2 = engine.forward(b2, mode=0, precision="float64")
+    oo, oo2 = b.out_off(), b2.out_off()
+    for new, k in enumerate(order):
+        np.testing.assert_array_equal(out2[oo2[new]:oo2[new + 1]], out[oo[k]:oo[k + 1]])

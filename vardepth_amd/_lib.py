@@ -1,0 +1,124 @@
+"""ctypes binding of libvdhmm.so (the C ABI declared in include/vdhmm.h).
+
+The library is built in-tree by ``python -m vardepth_amd.build``.  There is
+no CPU fallback: if the library is missing or no HIP device is present, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libvdhmm.so"
+CORE_LIB_PATH = PKG / "libvdhmm_core.so"
+
+_c_char_pp = ctypes.POINTER(ctypes.c_char_p)
+_u8p = ctypes.POINTER(ctypes.c_ubyte)
+_u8pp = ctypes.POINTER(_u8p)
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+# every extern "C" symbol include/vdhmm.h declares
+SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch",
+           "read_haplotype_batch_mode", "wide_capability", "anchored_pairhmm",
+           "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
+           "vd_forward", "vd_forward_device", "vd_last_error", "vd_device")
+
+
+class VdParams(ctypes.Structure):
+    _fields_ = [("gap", ctypes.c_double), ("ext", ctypes.c_double), ("err", ctypes.c_double * 64)]
+
+
+class VdBatch(ctypes.Structure):
+    _fields_ = [("read_bytes", ctypes.c_void_p), ("qual_bytes", ctypes.c_void_p),
+                ("read_off", ctypes.c_void_p), ("hap_bytes", ctypes.c_void_p),
+                ("hap_off", ctypes.c_void_p), ("region_reads", ctypes.c_void_p),
+                ("region_haps", ctypes.c_void_p), ("out_off", ctypes.c_void_p),
+                ("nregions", ctypes.c_int64), ("nreads", ctypes.c_int64), ("nhaps", ctypes.c_int64)]
+
+
+class VdStats(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("fp32_units", ctypes.c_uint64),
+                ("flagged", ctypes.c_uint64), ("fp64_pairs", ctypes.c_uint64), ("fp64_bad", ctypes.c_uint64),
+                ("overflow", ctypes.c_uint64), ("device_ms", ctypes.c_double), ("plan_ms", ctypes.c_double),
+                ("fwd32_ms", ctypes.c_double), ("fwd64_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class VdhmmError(RuntimeError):
+    pass
+
+
+@functools.lru_cache(maxsize=1)
+def lib() -> ctypes.CDLL:
+    """Load libvdhmm.so and declare argtypes (raises if it has not been built)."""
+    if not LIB_PATH.exists():
+        raise VdhmmError("libvdhmm.so is not built: run `python -m vardepth_amd.build` "
+                         "(the HIP engine has no CPU fallback)")
+    L = ctypes.CDLL(str(LIB_PATH))
+    L.pairhmm_pairs.argtypes = [_c_char_pp, _u8pp, _ip, _c_char_pp, _ip, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, _dp]
+    L.pairhmm_pairs.restype = ctypes.c_int
+    L.trie_many.argtypes = [_c_char_pp, _u8pp, _ip, ctypes.c_int, _c_char_pp, _ip, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, _dp, _u64p]
+    L.trie_many.restype = ctypes.c_int
+    args7 = [ctypes.c_char_p, _u8p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+             ctypes.c_double]
+    batch = [ctypes.c_char_p, _u8p, ctypes.c_int, _c_char_pp, _ip, ctypes.c_int, ctypes.c_double,
+             ctypes.c_double, _dp]
+    for name in ("read_haplotype", "anchored_pairhmm", "vd_core_read_haplotype"):
+        getattr(L, name).argtypes = args7
+        getattr(L, name).restype = ctypes.c_double
+    for name in ("read_haplotype_batch", "vd_core_read_haplotype_batch"):
+        getattr(L, name).argtypes = batch
+        getattr(L, name).restype = None
+    L.read_haplotype_batch_mode.argtypes = batch + [ctypes.c_int]
+    L.read_haplotype_batch_mode.restype = None
+    L.wide_capability.argtypes = []
+    L.wide_capability.restype = ctypes.c_int
+    L.vd_params_default.argtypes = [ctypes.POINTER(VdParams), ctypes.c_double, ctypes.c_double]
+    L.vd_params_default.restype = None
+    L.vd_forward.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int, ctypes.c_int,
+                             _dp, ctypes.POINTER(VdStats)]
+    L.vd_forward.restype = ctypes.c_int
+    L.vd_forward_device.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.POINTER(VdStats)]
+    L.vd_forward_device.restype = ctypes.c_int
+    L.vd_last_error.argtypes = []
+    L.vd_last_error.restype = ctypes.c_char_p
+    L.vd_device.argtypes = []
+    L.vd_device.restype = ctypes.c_int
+    return L
+
+
+def last_error() -> str:
+    return lib().vd_last_error().decode(errors="replace")
+
+
+def check(code: int, what: str):
+    if code:
+        raise VdhmmError("%s failed (%d): %s" % (what, code, last_error()))
+
+
+def params(gap: float, ext: float, err=None) -> VdParams:
+    """vd_params with the reference error table (or a 64-bin / 256-entry table)."""
+    p = VdParams()
+    lib().vd_params_default(ctypes.byref(p), float(gap), float(ext))
+    if err is not None:
+        e = np.asarray(err, dtype=np.float64)
+        if e.shape == (256,):
+            e = e[:64]
+        if e.shape != (64,):
+            raise ValueError("error table must have 64 bins (or 256 raw-quality entries)")
+        for i in range(64):
+            p.err[i] = float(e[i])
+    return p

@@ -1,0 +1,509 @@
+// Host side of the MI355X PairHMM engine: the C ABI of include/vdhmm.h.
+//
+// One process-wide engine (device, stream, grow-only device buffers) behind a
+// mutex.  vd_forward_device enqueues the whole batch -- planning, FP32
+// persistent kernel, FP64 rescue -- on one stream with no host round trip, so
+// a caller can capture it in a hipGraph once its buffers exist.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "vdhmm.h"
+#include "vdhmm_internal.h"
+
+namespace {
+
+std::mutex g_mu;
+std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return 2;
+}
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+struct Engine {
+  int dev = -1;
+  int cus = 0;
+  hipStream_t own_stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // staged inputs / outputs of the host entry points
+  DevBuf in_read, in_qual, in_roff, in_hap, in_hoff, in_rreg, in_hreg, in_ooff, in_out;
+  // plan scratch
+  DevBuf read_cls, read_exp, class_off, read_perm, unit_cnt, unit_bucket, f64_list, ctr, ws64;
+  // parameter tables
+  DevBuf qE, tab32, tab64;
+  vd_params params_on_device;
+  bool have_params = false;
+  int occ32[2] = {0, 0}, occ64[2] = {0, 0};
+  int occ_smax = -1;
+};
+
+Engine g;
+
+int engine_init() {
+  if (g.dev >= 0) return 0;
+  const char* env = std::getenv("VARDEPTH_CUDA_DEVICE");
+  int dev = env ? std::atoi(env) : 0;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0)
+    return fail(std::string("vdhmm: no HIP device available (") + hipGetErrorString(e) + ")");
+  if (dev < 0 || dev >= ndev) return fail("vdhmm: VARDEPTH_CUDA_DEVICE out of range");
+  HIPCHK(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  g.cus = prop.multiProcessorCount;
+  HIPCHK(hipStreamCreateWithFlags(&g.own_stream, hipStreamNonBlocking));
+  for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
+  g.dev = dev;
+  return 0;
+}
+
+template <typename T>
+void build_tab(const vd_params& p, vd::QTab<T>& t, int32_t* E) {
+  const double a = 1 - 2 * p.gap, b = 1 - p.ext, c = 0.25 * p.gap, d = 0.25 * p.ext;
+  for (int q = 0; q < 64; ++q) {
+    const double e = p.err[q];
+    const double gd = 1 - e, bd = e / 3;
+    const double best = std::max(gd, bd) * a;
+    int ex = 0;
+    if (best > 0) std::frexp(best, &ex);  // best = f 2^ex, f in [0.5, 1)
+    const double rho = std::ldexp(1.0, ex);
+    t.good[q] = (T)(gd * a / rho);
+    t.bad[q] = (T)(bd * a / rho);
+    t.mu[q] = (T)(d / rho);
+    t.beta[q] = (T)((b * c / a) / rho);
+    t.cfin[q] = (T)(c / rho);
+    t.E[q] = ex;
+    E[q] = ex;
+  }
+  t.gamma = (T)(b * p.gap / a);
+  t.gap = (T)p.gap;
+  t.ext = (T)p.ext;
+}
+
+int check_params(const vd_params& p) {
+  if (!(p.gap > 0 && p.gap < 0.5)) return fail("vdhmm: gap open must be in (0, 0.5)");
+  if (!(p.ext >= 0 && p.ext < 1)) return fail("vdhmm: gap continuation must be in [0, 1)");
+  for (int q = 0; q < 64; ++q)
+    if (!(p.err[q] >= 0 && p.err[q] <= 1)) return fail("vdhmm: error probabilities must be in [0, 1]");
+  return 0;
+}
+
+int upload_params(const vd_params& p, hipStream_t st) {
+  if (g.have_params && std::memcmp(&p, &g.params_on_device, sizeof(p)) == 0) return 0;
+  static vd::QTab<float> t32;
+  static vd::QTab<double> t64;
+  static int32_t E[64];
+  build_tab(p, t32, E);
+  build_tab(p, t64, E);
+  HIPCHK(g.qE.ensure(sizeof(E)));
+  HIPCHK(g.tab32.ensure(sizeof(t32)));
+  HIPCHK(g.tab64.ensure(sizeof(t64)));
+  HIPCHK(hipMemcpyAsync(g.qE.p, E, sizeof(E), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.tab32.p, &t32, sizeof(t32), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.tab64.p, &t64, sizeof(t64), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  g.params_on_device = p;
+  g.have_params = true;
+  return 0;
+}
+
+int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
+                          int max_hap_len, double* dout, hipStream_t st, vd_stats* stats) {
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (precision != VD_PREC_MIXED && precision != VD_PREC_FP64) return fail("vdhmm: bad precision");
+  if (db->nregions < 0 || db->nreads < 0 || db->nhaps < 0 || npairs < 0 || max_hap_len < 0)
+    return fail("vdhmm: negative batch size");
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  if (int rc = upload_params(*params, st)) return rc;
+  const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
+  const int force64 = precision == VD_PREC_FP64 ? 1 : 0;
+  const int smax = ((max_hap_len + 48) + 7) & ~7;
+  // plan buffers
+  const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
+  const int64_t nb = db->nregions * vd::NCLS;
+  const int64_t max_units = npairs / vd::PAIRS_PER_UNIT + nb + 1;
+  HIPCHK(g.read_cls.ensure(nreads));
+  HIPCHK(g.read_exp.ensure(nreads * 4));
+  HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
+  HIPCHK(g.read_perm.ensure(nreads * 8));
+  HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
+  HIPCHK(g.unit_bucket.ensure(max_units * 4));
+  HIPCHK(g.f64_list.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
+  HIPCHK(g.ctr.ensure(vd::CTR_COUNT * 8));
+  if (g.occ_smax != smax) {
+    for (int a = 0; a < 2; ++a) {
+      HIPCHK(vd::occupancy(a, 0, smax, &g.occ32[a]));
+      HIPCHK(vd::occupancy(a, 1, smax, &g.occ64[a]));
+    }
+    g.occ_smax = smax;
+  }
+  const int blocks32 = std::max(1, g.cus * std::max(1, g.occ32[anchored]));
+  const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 4)));
+  HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
+  vd::BatchDev b{db->read_bytes, db->qual_bytes, db->read_off, db->hap_bytes, db->hap_off,
+                 db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};
+  vd::PlanDev p{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
+                g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
+                g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1)};
+  HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
+                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks64,
+                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr));
+  if (stats) {
+    HIPCHK(hipEventSynchronize(g.ev[3]));
+    unsigned long long c[vd::CTR_COUNT];
+    HIPCHK(hipMemcpy(c, g.ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+    float ms = 0, a = 0, b2 = 0, c3 = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g.ev[0], g.ev[3]));
+    HIPCHK(hipEventElapsedTime(&a, g.ev[0], g.ev[1]));
+    HIPCHK(hipEventElapsedTime(&b2, g.ev[1], g.ev[2]));
+    HIPCHK(hipEventElapsedTime(&c3, g.ev[2], g.ev[3]));
+    stats->plan_ms = a;
+    stats->fwd32_ms = b2;
+    stats->fwd64_ms = c3;
+    stats->pairs = (uint64_t)npairs;
+    stats->fp32_units = force64 ? 0 : c[vd::CTR_NUNITS];
+    stats->flagged = c[vd::CTR_FLAGGED];
+    stats->fp64_pairs = std::min<unsigned long long>(c[vd::CTR_F64], (unsigned long long)std::max<int64_t>(npairs, 1));
+    stats->fp64_bad = c[vd::CTR_F64BAD];
+    stats->overflow = c[vd::CTR_OVERFLOW];
+    stats->device_ms = ms;
+    if (c[vd::CTR_OVERFLOW]) return fail("vdhmm: FP64 work list overflow");
+  }
+  return 0;
+}
+
+int validate_host_batch(const vd_batch* b, int64_t* npairs, int* max_hap, uint64_t* cells) {
+  if (b->nregions < 0 || b->nreads < 0 || b->nhaps < 0) return fail("vdhmm: negative batch size");
+  if (b->read_off[0] != 0 || b->hap_off[0] != 0 || b->region_reads[0] != 0 || b->region_haps[0] != 0 ||
+      b->out_off[0] != 0)
+    return fail("vdhmm: offsets must start at 0");
+  for (int64_t i = 0; i < b->nreads; ++i)
+    if (b->read_off[i + 1] < b->read_off[i]) return fail("vdhmm: read offsets must be non-decreasing");
+  int mh = 0;
+  for (int64_t i = 0; i < b->nhaps; ++i) {
+    int64_t len = b->hap_off[i + 1] - b->hap_off[i];
+    if (len < 0) return fail("vdhmm: haplotype offsets must be non-decreasing");
+    if (len > (1 << 20)) return fail("vdhmm: haplotype longer than 2^20 bases");
+    mh = std::max<int>(mh, (int)len);
+  }
+  uint64_t cl = 0;
+  for (int64_t k = 0; k < b->nregions; ++k) {
+    int64_t r0 = b->region_reads[k], r1 = b->region_reads[k + 1];
+    int64_t h0 = b->region_haps[k], h1 = b->region_haps[k + 1];
+    if (r1 < r0 || h1 < h0 || r1 > b->nreads || h1 > b->nhaps) return fail("vdhmm: bad region ranges");
+    if (b->out_off[k + 1] - b->out_off[k] != (r1 - r0) * (h1 - h0))
+      return fail("vdhmm: out_off does not match region sizes");
+    cl += (uint64_t)(b->read_off[r1] - b->read_off[r0]) * (uint64_t)(b->hap_off[h1] - b->hap_off[h0]);
+  }
+  if (b->region_reads[b->nregions] != b->nreads || b->region_haps[b->nregions] != b->nhaps)
+    return fail("vdhmm: regions must cover all reads and haplotypes");
+  *npairs = b->out_off[b->nregions];
+  *max_hap = mh;
+  *cells = cl;
+  return 0;
+}
+
+int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, int precision, double* out,
+                        vd_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  int64_t npairs = 0;
+  int max_hap = 0;
+  uint64_t cells = 0;
+  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  const int64_t nrb = hb->read_off[hb->nreads], nhb = hb->hap_off[hb->nhaps];
+  HIPCHK(g.in_read.ensure(nrb + 1));
+  HIPCHK(g.in_qual.ensure(nrb + 1));
+  HIPCHK(g.in_roff.ensure((hb->nreads + 1) * 8));
+  HIPCHK(g.in_hap.ensure(nhb + 1));
+  HIPCHK(g.in_hoff.ensure((hb->nhaps + 1) * 8));
+  HIPCHK(g.in_rreg.ensure((hb->nregions + 1) * 8));
+  HIPCHK(g.in_hreg.ensure((hb->nregions + 1) * 8));
+  HIPCHK(g.in_ooff.ensure((hb->nregions + 1) * 8));
+  HIPCHK(g.in_out.ensure(std::max<int64_t>(npairs, 1) * 8));
+  if (nrb) {
+    HIPCHK(hipMemcpyAsync(g.in_read.p, hb->read_bytes, nrb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(g.in_qual.p, hb->qual_bytes, nrb, hipMemcpyHostToDevice, st));
+  }
+  if (nhb) HIPCHK(hipMemcpyAsync(g.in_hap.p, hb->hap_bytes, nhb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.in_roff.p, hb->read_off, (hb->nreads + 1) * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.in_hoff.p, hb->hap_off, (hb->nhaps + 1) * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.in_rreg.p, hb->region_reads, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.in_hreg.p, hb->region_haps, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g.in_ooff.p, hb->out_off, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
+  vd_batch db{g.in_read.as<uint8_t>(), g.in_qual.as<uint8_t>(), g.in_roff.as<int64_t>(), g.in_hap.as<uint8_t>(),
+              g.in_hoff.as<int64_t>(), g.in_rreg.as<int64_t>(), g.in_hreg.as<int64_t>(),
+              g.in_ooff.as<int64_t>(), hb->nregions, hb->nreads, hb->nhaps};
+  vd_stats local;
+  vd_stats* s = stats ? stats : &local;
+  std::memset(s, 0, sizeof(*s));
+  if (int rc = forward_device_locked(&db, params, mode, precision, npairs, max_hap, g.in_out.as<double>(), st, s))
+    return rc;
+  if (npairs) HIPCHK(hipMemcpyAsync(out, g.in_out.p, npairs * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s->cells = cells;
+  s->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (s->fp64_bad) return fail("vdhmm: FP64 pass produced non-finite likelihoods");
+  return 0;
+}
+
+// ---- helpers for the drop-ins ---------------------------------------------
+
+struct HostBatch {
+  std::vector<uint8_t> rb, qb, hb;
+  std::vector<int64_t> ro{0}, ho{0}, rr{0}, rh{0}, oo{0};
+  void add_read(const char* s, const unsigned char* q, int n) {
+    if (n > 0) {
+      rb.insert(rb.end(), (const uint8_t*)s, (const uint8_t*)s + n);
+      qb.insert(qb.end(), q, q + n);
+    }
+    ro.push_back(ro.back() + std::max(n, 0));
+  }
+  void add_hap(const char* h, int m) {
+    if (m > 0) hb.insert(hb.end(), (const uint8_t*)h, (const uint8_t*)h + m);
+    ho.push_back(ho.back() + std::max(m, 0));
+  }
+  void close_region() {
+    int64_t R = (int64_t)ro.size() - 1 - rr.back(), H = (int64_t)ho.size() - 1 - rh.back();
+    rr.push_back((int64_t)ro.size() - 1);
+    rh.push_back((int64_t)ho.size() - 1);
+    oo.push_back(oo.back() + R * H);
+  }
+  vd_batch view() const {
+    return vd_batch{rb.data(), qb.data(), ro.data(), hb.data(), ho.data(), rr.data(), rh.data(), oo.data(),
+                    (int64_t)rr.size() - 1, (int64_t)ro.size() - 1, (int64_t)ho.size() - 1};
+  }
+};
+
+int run_host(const HostBatch& hbat, double gap, double ext, int mode, int precision, double* out) {
+  vd_params p;
+  vd_params_default(&p, gap, ext);
+  vd_batch v = hbat.view();
+  return forward_host_locked(&v, &p, mode, precision, out, nullptr);
+}
+
+int env_precision() {
+  const char* e = std::getenv("VARDEPTH_CUDA_PRECISION");
+  if (e && std::strcmp(e, "float64") == 0) return VD_PREC_FP64;
+  return VD_PREC_MIXED;
+}
+
+// trie_vector.cpp:198-205,271: evaluated-cell counter of the reference's
+// prefix-sharing kernel (lane width 8/4/2 as dispatched on an AVX-512 host).
+uint64_t trie_cells(const int* rn, int nr, const char* const* haps, const int* hn, int nh, int lanes) {
+  int suffix = 0;
+  if (nh > 1) {
+    int minimum = *std::min_element(hn, hn + nh);
+    while (suffix < minimum) {
+      char c = haps[0][hn[0] - suffix - 1];
+      bool same = true;
+      for (int h = 1; h < nh; ++h)
+        if (haps[h][hn[h] - suffix - 1] != c) {
+          same = false;
+          break;
+        }
+      if (!same) break;
+      ++suffix;
+    }
+    if (suffix < 16) suffix = 0;
+  }
+  uint64_t columns = 0;
+  for (int h = 0; h < nh; ++h) {
+    int shared = 0;
+    if (h) {
+      int lim = std::min(hn[h], hn[h - 1]) - suffix;
+      while (shared < lim && haps[h][shared] == haps[h - 1][shared]) ++shared;
+    }
+    int length = hn[h] - suffix;
+    if (length > shared) columns += (uint64_t)(length - shared);
+  }
+  const int L = (lanes == 0 || lanes == 8) ? 8 : (lanes == 4 ? 4 : 2);
+  uint64_t cells = 0;
+  for (int first = 0; first < nr; first += L) {
+    int take = std::min(L, nr - first), n = 0;
+    for (int a = 0; a < take; ++a) n = std::max(n, rn[first + a]);
+    cells += (uint64_t)n * (uint64_t)take * columns;
+  }
+  return cells;
+}
+
+double single(const char* read, const unsigned char* qual, int n, const char* hap, int m, double gap, double ext,
+              int mode) {
+  if (n <= 0 || m <= 0) return -INFINITY;
+  HostBatch hb;
+  hb.add_read(read, qual, n);
+  hb.add_hap(hap, m);
+  hb.close_region();
+  double v = NAN;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (run_host(hb, gap, ext, mode, VD_PREC_FP64, &v)) {
+    std::fprintf(stderr, "%s\n", g_err.c_str());
+    return NAN;
+  }
+  return v;
+}
+
+void one_read_batch(const char* read, const unsigned char* qual, int n, const char* const* haps, const int* lengths,
+                    int count, double gap, double ext, double* out, int mode) {
+  if (count <= 0) return;
+  HostBatch hb;
+  hb.add_read(read, qual, n);
+  for (int k = 0; k < count; ++k) hb.add_hap(haps[k], lengths[k]);
+  hb.close_region();
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (run_host(hb, gap, ext, mode, VD_PREC_FP64, out)) {
+    std::fprintf(stderr, "%s\n", g_err.c_str());
+    for (int k = 0; k < count; ++k) out[k] = NAN;
+  }
+}
+
+}  // namespace
+
+// ---- C ABI ------------------------------------------------------------------
+
+extern "C" {
+
+void vd_params_default(vd_params* p, double gap, double ext) {
+  p->gap = gap;
+  p->ext = ext;
+  // batch_hmm.cpp:14: 10^(-min(q,60)/10) (bins 60..63 all equal the q>=60 value)
+  for (int q = 0; q < 64; ++q) p->err[q] = std::pow(10., -std::min(q, 60) / 10.);
+}
+
+int vd_forward(const vd_batch* batch, const vd_params* params, int mode, int precision, double* out,
+               vd_stats* stats) {
+  if (!batch || !params || !out) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return forward_host_locked(batch, params, mode, precision, out, stats);
+}
+
+int vd_forward_device(const vd_batch* dev_batch, const vd_params* params, int mode, int precision, int64_t npairs,
+                      int max_hap_len, double* dev_out, void* stream, vd_stats* stats) {
+  if (!dev_batch || !params || !dev_out) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (int rc = engine_init()) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc = forward_device_locked(dev_batch, params, mode, precision, npairs, max_hap_len, dev_out,
+                                 (hipStream_t)stream, stats);
+  if (stats && !rc)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+const char* vd_last_error(void) { return g_err.c_str(); }
+
+int vd_device(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return engine_init() ? -1 : g.dev;
+}
+
+int pairhmm_pairs(const char* const* reads, const unsigned char* const* quals, const int* rn,
+                  const char* const* haps, const int* hn, int count, int mode, int lanes, double* output) {
+  (void)lanes;
+  if (count < 0 || mode < 0 || mode > 2) return 1;  // batch_hmm.cpp:194
+  if (count == 0) return 0;
+  HostBatch hb;
+  for (int p = 0; p < count; ++p) {
+    hb.add_read(reads[p], quals[p], rn[p]);
+    hb.add_hap(haps[p], hn[p]);
+    hb.close_region();
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  return run_host(hb, 0.001, 0.1, mode, VD_PREC_FP64, output) ? 2 : 0;
+}
+
+int trie_many(const char* const* reads, const unsigned char* const* quals, const int* rn, int nr,
+              const char* const* haps, const int* hn, int nh, int mode, int lanes, double* out, uint64_t* cells) {
+  if (nr < 0 || nh < 0 || mode < 0 || mode > 2) return 1;  // trie_vector.cpp:286
+  *cells = trie_cells(rn, nr, haps, hn, nh, lanes);
+  if (nr == 0 || nh == 0) return 0;
+  HostBatch hb;
+  for (int a = 0; a < nr; ++a) hb.add_read(reads[a], quals[a], rn[a]);
+  for (int h = 0; h < nh; ++h) hb.add_hap(haps[h], hn[h]);
+  hb.close_region();
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (run_host(hb, 0.001, 0.1, mode, env_precision(), out)) return 2;
+  }
+  // trie_vector.cpp:277: -inf for empty inputs, NaN when the likelihood is <= 1e-200
+  const double lim = std::log(1e-200);
+  for (int64_t i = 0; i < (int64_t)nr * nh; ++i)
+    if (!std::isinf(out[i]) && !(out[i] > lim)) out[i] = NAN;
+  return 0;
+}
+
+double read_haplotype(const char* read, const unsigned char* qual, int n, const char* hap, int m, double gap,
+                      double ext) {
+  return single(read, qual, n, hap, m, gap, ext, VD_MODE_SEMIGLOBAL_RESCALE);
+}
+
+void read_haplotype_batch(const char* read, const unsigned char* qual, int n, const char* const* hap,
+                          const int* lengths, int count, double gap, double ext, double* out) {
+  one_read_batch(read, qual, n, hap, lengths, count, gap, ext, out, VD_MODE_SEMIGLOBAL_RESCALE);
+}
+
+void read_haplotype_batch_mode(const char* read, const unsigned char* qual, int n, const char* const* haps,
+                               const int* lengths, int count, double gap, double ext, double* out, int mode) {
+  (void)mode;
+  one_read_batch(read, qual, n, haps, lengths, count, gap, ext, out, VD_MODE_SEMIGLOBAL_RESCALE);
+}
+
+int wide_capability(void) { return 8; }
+
+double anchored_pairhmm(const char* read, const unsigned char* qual, int n, const char* hap, int m, double gap,
+                        double extension) {
+  return single(read, qual, n, hap, m, gap, extension, VD_MODE_ANCHORED);
+}
+
+double vd_core_read_haplotype(const char* read, const unsigned char* qual, int n, const char* hap, int m, double gap,
+                              double ext) {
+  return single(read, qual, n, hap, m, gap, ext, VD_MODE_SEMIGLOBAL);
+}
+
+void vd_core_read_haplotype_batch(const char* read, const unsigned char* qual, int n, const char* const* hap,
+                                  const int* lengths, int count, double gap, double ext, double* out) {
+  one_read_batch(read, qual, n, hap, lengths, count, gap, ext, out, VD_MODE_SEMIGLOBAL);
+}
+
+}  // extern "C"

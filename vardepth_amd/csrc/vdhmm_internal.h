@@ -1,0 +1,103 @@
+// Internal types shared by the HIP kernels (vdhmm_kernels.hip) and the host
+// engine (vdhmm_engine.cpp).  Not part of the public C-ABI (include/vdhmm.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vd {
+
+// Read classes.  Class c in [1,16]: the FP32 anti-diagonal kernel with c rows
+// per lane (a 16-lane group covers 16*c rows).  CLS_EMPTY: read of length 0
+// (every pair is -inf).  CLS_F64: rows beyond 256, or FP64 requested -> the
+// FP64 row-tiled kernel.
+constexpr int CLS_EMPTY = 0;
+constexpr int CLS_F64 = 17;
+constexpr int NCLS = 18;
+constexpr int MAX_R32 = 16;
+
+constexpr int GROUP = 16;        // lanes per pair (one DPP row)
+constexpr int PAIRS_PER_UNIT = 4;
+constexpr int R64 = 4;           // FP64 rows per lane -> 64-row tiles
+
+// hap code stream sentinels (hap bytes are 0..255)
+constexpr uint32_t SENT_POST = 0x100u;  // column beyond the haplotype end
+constexpr uint32_t SENT_PRE = 0x101u;   // column <= 0
+constexpr uint32_t NOMATCH = 0xFFFFu;   // read code of dummy rows
+
+constexpr int CEXP32 = 96;    // FP32 start scale 2^96
+constexpr int CEXP64 = 500;   // FP64 start / tile-renormalised scale 2^500
+constexpr float FLAG32_MIN = 0x1p-90f;  // FP32 terminal below this -> FP64 rescue
+
+// Per-quality-bin row constants of the rescaled recurrence, for one
+// precision.  With a = 1-2g, b = 1-x, c = g/4, d = x/4 and a per-bin
+// power-of-two normaliser rho_q = 2^E_q >= max(1-e, e/3) * a:
+//   good = (1-e) a / rho, bad = (e/3) a / rho     (match emission, scaled)
+//   mu   = d / rho                                 (I recurrence of the next row)
+//   beta = (b c / a) / rho                         (I term of the next row's M)
+//   cfin = c / rho                                 (terminal weight of I)
+//   gamma = b g / a                                (D term of the next row's M)
+template <typename T>
+struct QTab {
+  T good[64], bad[64], mu[64], beta[64], cfin[64];
+  int E[64];
+  T gamma, gap, ext;
+};
+
+// One read x haplotype pair addressed by global read / hap index.
+struct PairRef {
+  int64_t read, hap, out;
+};
+
+// Device view of a region batch (CSR, see vardepth_amd/workloads.py).
+struct BatchDev {
+  const uint8_t* read_bytes;
+  const uint8_t* qual_bytes;
+  const int64_t* read_off;
+  const uint8_t* hap_bytes;
+  const int64_t* hap_off;
+  const int64_t* region_reads;
+  const int64_t* region_haps;
+  const int64_t* out_off;
+  int64_t nregions, nreads, nhaps;
+};
+
+// Device scratch of one planned batch.
+struct PlanDev {
+  int8_t* read_cls;         // [nreads]
+  int32_t* read_exp;        // [nreads]  sum of E_q over the read
+  int32_t* class_off;       // [nregions * (NCLS+1)] per-region class offsets
+  int64_t* read_perm;       // [nreads]  reads grouped by class inside each region
+  int64_t* unit_cnt;        // [nregions * NCLS + 1] FP32 units per bucket, scanned in place
+  int32_t* unit_bucket;     // [total units] bucket of every unit
+  PairRef* f64_list;        // [capacity] FP64 work list
+  unsigned long long* ctr;  // counters, see CTR_*
+  int64_t f64_cap;
+};
+
+enum {
+  CTR_UNIT32 = 0,   // next FP32 unit
+  CTR_F64 = 1,      // FP64 list length
+  CTR_UNIT64 = 2,   // next FP64 unit
+  CTR_FLAGGED = 3,  // FP32 pairs flagged for rescue
+  CTR_OVERFLOW = 4, // FP64 list overflow (pairs dropped: reported as error)
+  CTR_NUNITS = 5,   // total FP32 units (written by the scan)
+  CTR_F64BAD = 6,   // FP64 results that were not finite and positive
+  CTR_COUNT = 8
+};
+
+struct LaunchCfg {
+  int blocks32;   // persistent FP32 grid (one wave per block)
+  int blocks64;   // persistent FP64 grid
+  int smax;       // LDS hap-code slots per group (>= max hap length + 40)
+};
+
+}  // namespace vd
+
+namespace vd {
+size_t lds_fwd32(int smax);
+size_t lds_fwd64(int smax);
+hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
+hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
+                           const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
+                           int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
+}  // namespace vd

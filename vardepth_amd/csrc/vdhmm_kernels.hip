@@ -1,0 +1,672 @@
+// MI355X (gfx950) PairHMM kernels.
+//
+// The forward recurrence of the reference (batch_hmm.cpp:31-70 dense form,
+// trie_vector.cpp:259-277 column form) is evaluated along anti-diagonals:
+//
+// * A 16-lane group (one DPP row) owns one read x haplotype pair; lane l holds
+//   R consecutive read rows, all at haplotype column t-l-1 at step t.  Rows
+//   inside a lane exchange values through registers; the row above a lane's
+//   first row lives in lane l-1 and arrives through a DPP row_shr:1 move whose
+//   `old` operand injects the boundary row into lane 0.  A wave runs 4 pairs.
+// * The recurrence is rescaled so that every cell costs 5 FMA/MUL plus the
+//   emission select: with per-row power-of-two normalisers rho_i,
+//     D^_i(j) = M^_i(j-1) + x D^_i(j-1)
+//     P_i(j)  = M^_i(j) + mu_{i+1} I^_i(j)        (= I^_{i+1}(j))
+//     Q_i(j)  = M^_i(j) + beta_{i+1} I^_i(j) + gamma D^_i(j)
+//     M^_i(j) = e'_ij Q_{i-1}(j-1)
+//   (M = kappa_i M^, I = c kappa_{i-1} I^, D = g kappa_i D^, kappa_i =
+//   kappa_{i-1} rho_i), which is the reference model exactly; only rounding
+//   differs.
+// * Reads are bottom-aligned in the group's 16R rows so that the last read
+//   row is always lane 15 / register R-1.  Rows above the read are
+//   pass-through rows (e'=0, mu=beta=1) that carry the injected boundary row
+//   down unchanged.  The semiglobal terminal sum_j (M+I)(n,j) is the last
+//   row's P stream (mu_last = c/rho_n) summed over the steps; columns past the
+//   haplotype end get zero emission so that the sum needs no masking.
+//   Anchored mode prepends a virtual row 0 (e'=1, mu=beta=0) that builds the
+//   reference's init row M(0,0)=1, D(0,j)=g x^(j-1).
+//
+// FP32 runs the bulk with a start scale of 2^96; a pair whose scaled
+// terminal falls below 2^-90 is appended to the FP64 work list.  The FP64
+// kernel uses the same recurrence with 64-row tiles, renormalising the
+// boundary row between tiles, so it covers any read length.
+#include "vdhmm_internal.h"
+
+namespace vd {
+
+#define LN2 0.69314718055994530942
+
+__device__ __forceinline__ float dpp_shr1(float old, float src) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
+                                         0x111, 0xF, 0xF, false));
+}
+
+struct I2 {
+  int lo, hi;
+};
+
+__device__ __forceinline__ double dpp_shr1(double old, double src) {
+  I2 o = __builtin_bit_cast(I2, old), s = __builtin_bit_cast(I2, src);
+  I2 r;
+  r.lo = __builtin_amdgcn_update_dpp(o.lo, s.lo, 0x111, 0xF, 0xF, false);
+  r.hi = __builtin_amdgcn_update_dpp(o.hi, s.hi, 0x111, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ---------------------------------------------------------------------------
+// planning
+
+// Class and exponent sum of every read.
+__global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, int anchored, int force64,
+                             int8_t* __restrict__ cls, int32_t* __restrict__ exps) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.nreads;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t o0 = b.read_off[r], o1 = b.read_off[r + 1];
+    int n = (int)(o1 - o0);
+    int e = 0;
+    for (int64_t k = o0; k < o1; ++k) {
+      int q = b.qual_bytes[k];
+      e += qE[q < 63 ? q : 63];
+    }
+    int np = n + anchored;
+    int c = n == 0 ? CLS_EMPTY : ((force64 || np > GROUP * MAX_R32) ? CLS_F64 : (np + GROUP - 1) / GROUP);
+    cls[r] = (int8_t)c;
+    exps[r] = e;
+  }
+}
+
+// One block per region: class histogram, per-region class offsets, the read
+// permutation grouped by class, FP32 unit counts, -inf for empty reads and the
+// FP64 list entries of long reads.
+__global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
+  __shared__ int cnt[NCLS], off[NCLS + 1], fill[NCLS];
+  __shared__ unsigned long long base64;
+  for (int64_t k = blockIdx.x; k < b.nregions; k += gridDim.x) {
+    if (threadIdx.x < NCLS) {
+      cnt[threadIdx.x] = 0;
+      fill[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int64_t r0 = b.region_reads[k], r1 = b.region_reads[k + 1];
+    const int64_t h0 = b.region_haps[k], H = b.region_haps[k + 1] - h0;
+    const int64_t o0 = b.out_off[k];
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      int c = p.read_cls[r];
+      atomicAdd(&cnt[c], 1);
+      if (c == CLS_EMPTY)
+        for (int64_t h = 0; h < H; ++h) out[o0 + (r - r0) * H + h] = -__builtin_inf();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      off[0] = 0;
+      for (int c = 0; c < NCLS; ++c) off[c + 1] = off[c] + cnt[c];
+      for (int c = 0; c <= NCLS; ++c) p.class_off[k * (NCLS + 1) + c] = off[c];
+      unsigned long long n64 = (unsigned long long)cnt[CLS_F64] * (unsigned long long)H;
+      base64 = n64 ? atomicAdd(&p.ctr[CTR_F64], n64) : 0ull;
+    }
+    __syncthreads();
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      int c = p.read_cls[r];
+      int s = atomicAdd(&fill[c], 1);
+      p.read_perm[r0 + off[c] + s] = r;
+      if (c == CLS_F64) {
+        for (int64_t h = 0; h < H; ++h) {
+          unsigned long long idx = base64 + (unsigned long long)s * H + h;
+          if ((int64_t)idx < p.f64_cap) {
+            p.f64_list[idx] = PairRef{r, h0 + h, o0 + (r - r0) * H + h};
+          } else {
+            atomicAdd(&p.ctr[CTR_OVERFLOW], 1ull);
+          }
+        }
+      }
+    }
+    if (threadIdx.x < NCLS) {
+      int c = threadIdx.x;
+      int64_t units = (c >= 1 && c <= MAX_R32) ? ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT : 0;
+      p.unit_cnt[k * NCLS + c] = units;
+    }
+    __syncthreads();
+  }
+}
+
+// Single-block exclusive scan of unit_cnt[0..n) in place; unit_cnt[n] and
+// ctr[CTR_NUNITS] receive the total.
+__global__ void k_scan_units(int64_t* __restrict__ a, int64_t n, unsigned long long* ctr) {
+  __shared__ int64_t part[1024];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int64_t chunk = (n + nt - 1) / nt;
+  const int64_t lo = tid * chunk, hi = lo + chunk < n ? lo + chunk : n;
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += a[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int d = 1; d < nt; d <<= 1) {
+    int64_t v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int64_t run = tid ? part[tid - 1] : 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    int64_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  if (tid == nt - 1) {
+    a[n] = part[nt - 1];
+    ctr[CTR_NUNITS] = (unsigned long long)part[nt - 1];
+  }
+}
+
+// unit -> bucket map by binary search over the scanned counts.
+__global__ void k_unit_map(const int64_t* __restrict__ scanned, int64_t nb, const unsigned long long* ctr,
+                           int32_t* __restrict__ unit_bucket) {
+  const int64_t total = (int64_t)ctr[CTR_NUNITS];
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < total;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = nb;  // find last b with scanned[b] <= u
+    while (hi - lo > 1) {
+      int64_t mid = (lo + hi) >> 1;
+      if (scanned[mid] <= u) lo = mid; else hi = mid;
+    }
+    unit_bucket[u] = (int32_t)lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the rescaled anti-diagonal step for R rows of one lane
+
+template <typename T, int R, bool TAIL>
+__device__ __forceinline__ T adiag_step(T (&M)[R], T (&D)[R], T (&Q)[R], const T (&gd)[R], const T (&bd)[R],
+                                        const T (&mu)[R], const T (&be)[R], const uint32_t (&rc)[R], T gam,
+                                        T gam_last, T xx, T Pin, T Qin, uint32_t code) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) D[r] = fmaT(xx, D[r], M[r]);
+  const bool post = TAIL && code == SENT_POST;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    T qup = r == 0 ? Qin : Q[r - 1];
+    T e = rc[r] == code ? gd[r] : bd[r];
+    if (TAIL) e = post ? T(0) : e;
+    M[r] = e * qup;
+  }
+  T Ih = Pin;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T P = fmaT(mu[r], Ih, M[r]);
+    Q[r] = fmaT(r == R - 1 ? gam_last : gam, D[r], fmaT(be[r], Ih, M[r]));
+    Ih = P;
+  }
+  return Ih;
+}
+
+// Per-lane description of the lane's group pair inside a unit.
+struct GroupPair {
+  int64_t read, hap, out;
+  int n, m;        // read / haplotype length (m = unit max for invalid groups)
+  int valid;
+  int m_min, m_max;  // over the unit's valid groups
+};
+
+// Fill the unit's hap-code slots: slot k of group g holds the code of column
+// k-17 (SENT_PRE for columns <= 0, SENT_POST past the end).
+__device__ __forceinline__ void fill_codes(const BatchDev& b, const GroupPair& gp, uint16_t* codes, int smax,
+                                           int nslots) {
+  const int lane = threadIdx.x & 63;
+  int64_t hb[PAIRS_PER_UNIT];
+  int hm[PAIRS_PER_UNIT];
+#pragma unroll
+  for (int g = 0; g < PAIRS_PER_UNIT; ++g) {
+    int64_t hoff = b.hap_off[gp.hap];
+    hb[g] = __shfl(hoff, g * GROUP);
+    hm[g] = __shfl(gp.valid ? gp.m : 0, g * GROUP);
+  }
+  for (int kk = lane; kk < nslots; kk += 64) {
+    const int col = kk - 17;
+#pragma unroll
+    for (int g = 0; g < PAIRS_PER_UNIT; ++g) {
+      uint32_t c = col <= 0 ? SENT_PRE : (col > hm[g] ? SENT_POST : (uint32_t)b.hap_bytes[hb[g] + col - 1]);
+      codes[g * smax + kk] = (uint16_t)c;
+    }
+  }
+}
+
+template <int R, bool ANCH>
+__device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
+                                    uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+  const int tend = gp.m_max + 16;
+  const int tmain = gp.m_min + 2;
+  fill_codes(b, gp, codes, smax, tend + 18);
+  // row constants
+  float gd[R], bd[R], mu[R], be[R];
+  uint32_t rc[R];
+  const int np = gp.n + (ANCH ? 1 : 0);
+  const int pad = GROUP * R - np;
+  const int64_t rb = b.read_off[gp.read];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int rel = l * R + r - pad;
+    float vg = 0.f, vb = 0.f, vm = 0.f, ve = 0.f;
+    uint32_t vc = NOMATCH;
+    if (gp.valid) {
+      if (rel < 0) {
+        vm = ve = 1.f;  // pass-through row
+      } else if (ANCH && rel == 0) {
+        vg = vb = 1.f;  // virtual init row
+      } else {
+        const int kk = rel - (ANCH ? 1 : 0);
+        int q = b.qual_bytes[rb + kk];
+        q = q < 63 ? q : 63;
+        vc = b.read_bytes[rb + kk];
+        vg = tab->good[q];
+        vb = tab->bad[q];
+        vm = tab->mu[q];
+        ve = tab->beta[q];
+        if (kk == gp.n - 1) vm = ve = tab->cfin[q];
+      }
+    }
+    gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
+  }
+  const float gam = tab->gamma, xx = tab->ext;
+  const float gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : 0.f) : gam;
+  __syncthreads();  // codes visible
+
+  float M[R], D[R], Q[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.f;
+  const float C = 0x1p96f;
+  float Plast = 0.f, Xprev = 0.f, acc = 0.f, rmax = 0.f, latch = 0.f;
+  const uint16_t* cl = codes + g * smax + 16 - l;
+  // step 0 (column -1 for lane 0): injection of the boundary row's first values
+  {
+    const float inj = ANCH ? C : 0.f;
+    float Pin = dpp_shr1(inj, Plast);
+    float Xn = dpp_shr1(inj, Q[R - 1]);
+    Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[0]);
+    Xprev = Xn;
+    if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
+  }
+  const float inj = ANCH ? 0.f : C;
+  // One step at a time; the loops are unrolled by two so that the loop-carried
+  // M/D/Q registers return to the same physical registers without copies.
+  auto main_step = [&](int tt) {
+    const uint32_t code = cl[tt];
+    float Pin = dpp_shr1(inj, Plast);
+    float Xn = dpp_shr1(inj, Q[R - 1]);
+    Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+    Xprev = Xn;
+    if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
+  };
+  auto tail_step = [&](int tt) {
+    const uint32_t code = cl[tt];
+    // past the haplotype end the boundary row injects nothing
+    const float pinj = (!ANCH && tt - 1 > gp.m) ? 0.f : inj;
+    float Pin = dpp_shr1(pinj, Plast);
+    float Xn = dpp_shr1(inj, Q[R - 1]);
+    Plast = adiag_step<float, R, true>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+    Xprev = Xn;
+    if (ANCH) {
+      rmax = fmaxf(rmax, Q[R - 1]);
+      if (tt - l - 1 == gp.m) latch = Q[R - 1];
+    } else {
+      acc += Plast;
+    }
+  };
+  int t = 1;
+  for (; t + 1 < tmain; t += 2) {
+    main_step(t);
+    main_step(t + 1);
+  }
+  if (t < tmain) main_step(t++);
+  for (; t + 1 <= tend; t += 2) {
+    tail_step(t);
+    tail_step(t + 1);
+  }
+  if (t <= tend) tail_step(t++);
+  if (l == GROUP - 1 && gp.valid) {
+    const double lnk = (double)(p.read_exp[gp.read] - CEXP32) * LN2;
+    double v;
+    bool flag;
+    if (ANCH) {
+      flag = !(latch >= FLAG32_MIN) || !(rmax < __builtin_inff());
+      double cl2 = 1e-300 * (double)rmax;
+      v = log((double)latch > cl2 ? (double)latch : cl2) + lnk;
+    } else {
+      flag = !(acc >= FLAG32_MIN) || !(acc < __builtin_inff());
+      v = log((double)acc) + lnk - log((double)(gp.m + 1));
+    }
+    if (flag) {
+      unsigned long long idx = atomicAdd(&p.ctr[CTR_F64], 1ull);
+      atomicAdd(&p.ctr[CTR_FLAGGED], 1ull);
+      if ((int64_t)idx < p.f64_cap) p.f64_list[idx] = PairRef{gp.read, gp.hap, gp.out};
+      else atomicAdd(&p.ctr[CTR_OVERFLOW], 1ull);
+      out[gp.out] = __builtin_nan("");
+    } else {
+      out[gp.out] = v;
+    }
+  }
+  __syncthreads();  // codes reused by the next unit
+}
+
+template <bool ANCH>
+__global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
+                                              int smax, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds);
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
+    for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const unsigned long long total = p.ctr[CTR_NUNITS];
+  for (;;) {
+    unsigned long long u = 0;
+    if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT32], 1ull);
+    u = __shfl(u, 0);
+    if (u >= total) break;
+    const int bkt = p.unit_bucket[u];
+    const int64_t j = (int64_t)u - p.unit_cnt[bkt];
+    const int64_t k = bkt / NCLS;
+    const int c = bkt % NCLS;
+    const int64_t r0 = b.region_reads[k], h0 = b.region_haps[k];
+    const int64_t H = b.region_haps[k + 1] - h0;
+    const int32_t* co = p.class_off + k * (NCLS + 1);
+    const int64_t cnt = co[c + 1] - co[c];
+    const int64_t pi = j * PAIRS_PER_UNIT + g;
+    GroupPair gp;
+    gp.valid = pi < cnt * H;
+    const int64_t s = gp.valid ? pi / H : 0, h = gp.valid ? pi % H : 0;
+    gp.read = p.read_perm[r0 + co[c] + s];
+    gp.hap = h0 + h;
+    gp.out = b.out_off[k] + (gp.read - r0) * H + h;
+    gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+    gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+    if (gp.valid && gp.m == 0) {
+      if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
+      gp.valid = 0;
+    }
+    int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
+    mx = max(mx, __shfl_xor(mx, 16));
+    mx = max(mx, __shfl_xor(mx, 32));
+    mn = min(mn, __shfl_xor(mn, 16));
+    mn = min(mn, __shfl_xor(mn, 32));
+    if (mx == 0) continue;
+    gp.m_max = mx;
+    gp.m_min = mn;
+    if (!gp.valid) gp.m = mx;
+    switch (c) {
+#define VD_CASE(RR) \
+  case RR: unit32<RR, ANCH>(b, p, tab, codes, smax, gp, out); break;
+      VD_CASE(1) VD_CASE(2) VD_CASE(3) VD_CASE(4) VD_CASE(5) VD_CASE(6) VD_CASE(7) VD_CASE(8)
+      VD_CASE(9) VD_CASE(10) VD_CASE(11) VD_CASE(12) VD_CASE(13) VD_CASE(14) VD_CASE(15) VD_CASE(16)
+#undef VD_CASE
+      default: break;
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// FP64: 64-row tiles (R64 rows per lane).  Tile boundaries go through a global
+// workspace (lane 15 writes the last row's P/Q streams, lane 0 of the next
+// tile injects them), renormalised to 2^CEXP64 so that any read length stays
+// inside the FP64 range.
+
+template <bool ANCH>
+__device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, const QTab<double>* __restrict__ tab,
+                                       uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ ws,
+                                       double* __restrict__ out) {
+  constexpr int R = R64;
+  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+  const int tend = gp.m_max + 16;
+  const int tmain = gp.m_min + 2;
+  const int nslots = tend + 18;
+  fill_codes(b, gp, codes, smax, nslots);
+  const int np = gp.n + (ANCH ? 1 : 0);
+  int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
+  int ntiles = max(ntile_g, __shfl_xor(ntile_g, 16));
+  ntiles = max(ntiles, __shfl_xor(ntiles, 32));
+  const int pad = ntiles * GROUP * R - np;
+  const int64_t rb = b.read_off[gp.read];
+  const double gam = tab->gamma, xx = tab->ext;
+  const double gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : 0.0) : gam;
+  const double C = 0x1p500;
+  // boundary buffers: [buffer][stream][group][slot], slot = column + 17
+  double* bufP[2];
+  double* bufQ[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    bufP[k] = ws + ((size_t)(2 * k + 0) * PAIRS_PER_UNIT + g) * smax;
+    bufQ[k] = ws + ((size_t)(2 * k + 1) * PAIRS_PER_UNIT + g) * smax;
+  }
+  int shift_total = 0;   // sum of tile renormalisation exponents (log2)
+  double sc = 1.0;       // scale applied to the injected boundary of this tile
+  double acc = 0.0, rmax = 0.0, latch = 0.0;
+  const uint16_t* cl = codes + g * smax + 16 - l;
+  __syncthreads();
+  for (int tile = 0; tile < ntiles; ++tile) {
+    const bool first = tile == 0, last = tile == ntiles - 1;
+    double gd[R], bd[R], mu[R], be[R];
+    uint32_t rc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int rel = tile * GROUP * R + l * R + r - pad;
+      double vg = 0.0, vb = 0.0, vm = 0.0, ve = 0.0;
+      uint32_t vc = NOMATCH;
+      if (gp.valid) {
+        if (rel < 0) {
+          vm = ve = 1.0;
+        } else if (ANCH && rel == 0) {
+          vg = vb = 1.0;
+        } else {
+          const int kk = rel - (ANCH ? 1 : 0);
+          int q = b.qual_bytes[rb + kk];
+          q = q < 63 ? q : 63;
+          vc = b.read_bytes[rb + kk];
+          vg = tab->good[q];
+          vb = tab->bad[q];
+          vm = tab->mu[q];
+          ve = tab->beta[q];
+          if (kk == gp.n - 1) vm = ve = tab->cfin[q];
+        }
+      }
+      gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
+    }
+    double M[R], D[R], Q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.0;
+    double Plast = 0.0, Xprev = 0.0, bmax = 0.0;
+    const double* inP = bufP[(tile + 1) & 1];
+    const double* inQ = bufQ[(tile + 1) & 1];
+    double* oP = bufP[tile & 1];
+    double* oQ = bufQ[tile & 1];
+    for (int t = 0; t <= tend; ++t) {
+      const uint32_t code = cl[t];
+      double pinj, qinj;
+      if (first) {
+        if (ANCH) {
+          pinj = qinj = t == 0 ? C : 0.0;
+        } else {
+          pinj = (t == 0 || t - 1 > gp.m) ? 0.0 : C;
+          qinj = t == 0 ? 0.0 : C;
+        }
+      } else {
+        // column t-1 of the previous tile's last row (slot t+16); none past m_max
+        const bool have = t - 1 <= gp.m_max;
+        pinj = have ? inP[t + 16] * sc : 0.0;
+        qinj = have ? inQ[t + 16] * sc : 0.0;
+      }
+      const double Pin = dpp_shr1(pinj, Plast);
+      const double Xn = dpp_shr1(qinj, Q[R - 1]);
+      if (t >= tmain)
+        Plast = adiag_step<double, R, true>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+      else
+        Plast = adiag_step<double, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+      Xprev = Xn;
+      if (last) {
+        if (ANCH) {
+          rmax = fmax(rmax, Q[R - 1]);
+          if (t - l - 1 == gp.m) latch = Q[R - 1];
+        } else {
+          acc += Plast;
+        }
+      } else if (l == GROUP - 1) {
+        const int slot = t + 1;  // column t-16
+        oP[slot] = Plast;
+        oQ[slot] = Q[R - 1];
+        bmax = fmax(bmax, fmax(Plast, Q[R - 1]));
+      }
+    }
+    if (!last) {
+      // renormalise the boundary row to 2^CEXP64
+      int e = 0;
+      if (bmax > 0.0) frexp(bmax, &e);
+      const int s = bmax > 0.0 ? CEXP64 - e : 0;
+      const double scl = ldexp(1.0, s);
+      sc = __shfl(scl, (g * GROUP) + GROUP - 1);
+      shift_total -= __shfl(s, (g * GROUP) + GROUP - 1);
+      __threadfence();
+      __syncthreads();
+    }
+  }
+  if (l == GROUP - 1 && gp.valid) {
+    const double lnk = (double)(p.read_exp[gp.read] - CEXP64 + shift_total) * LN2;
+    double v;
+    bool bad;
+    if (ANCH) {
+      const double cl2 = 1e-300 * rmax;
+      bad = !(rmax > 0.0) || !(rmax < __builtin_inf());
+      v = log(latch > cl2 ? latch : cl2) + lnk;
+    } else {
+      bad = !(acc > 0.0) || !(acc < __builtin_inf());
+      v = log(acc) + lnk - log((double)(gp.m + 1));
+    }
+    if (bad) atomicAdd(&p.ctr[CTR_F64BAD], 1ull);
+    out[gp.out] = v;
+  }
+  __syncthreads();
+}
+
+template <bool ANCH>
+__global__ __launch_bounds__(64) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
+                                              int smax, double* __restrict__ wsall, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<double>) + 15) & ~size_t(15)));
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
+    for (int i = threadIdx.x; i < (int)(sizeof(QTab<double>) / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  double* ws = wsall + (size_t)blockIdx.x * 4 * PAIRS_PER_UNIT * smax;
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  unsigned long long count = p.ctr[CTR_F64];
+  if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
+  const unsigned long long units = (count + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
+  for (;;) {
+    unsigned long long u = 0;
+    if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT64], 1ull);
+    u = __shfl(u, 0);
+    if (u >= units) break;
+    const unsigned long long idx = u * PAIRS_PER_UNIT + g;
+    GroupPair gp;
+    gp.valid = idx < count;
+    const PairRef pr = p.f64_list[gp.valid ? idx : u * PAIRS_PER_UNIT];
+    gp.read = pr.read;
+    gp.hap = pr.hap;
+    gp.out = pr.out;
+    gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+    gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+    if (gp.valid && gp.m == 0) {
+      if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
+      gp.valid = 0;
+    }
+    int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
+    mx = max(mx, __shfl_xor(mx, 16));
+    mx = max(mx, __shfl_xor(mx, 32));
+    mn = min(mn, __shfl_xor(mn, 16));
+    mn = min(mn, __shfl_xor(mn, 32));
+    if (mx == 0) continue;
+    gp.m_max = mx;
+    gp.m_min = mn;
+    if (!gp.valid) gp.m = mx;
+    unit64<ANCH>(b, p, tab, codes, smax, gp, ws, out);
+  }
+}
+
+template __global__ void k_fwd64<false>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
+template __global__ void k_fwd64<true>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
+
+template __global__ void k_fwd32<false>(BatchDev, PlanDev, const QTab<float>*, int, double*);
+template __global__ void k_fwd32<true>(BatchDev, PlanDev, const QTab<float>*, int, double*);
+
+}  // namespace vd
+
+// ---------------------------------------------------------------------------
+// host-side launchers (called by vdhmm_engine.cpp)
+
+namespace vd {
+
+__global__ void k_zero_ctr(unsigned long long* ctr, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) ctr[i] = 0ull;
+}
+
+size_t lds_fwd32(int smax) {
+  return ((sizeof(QTab<float>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
+}
+size_t lds_fwd64(int smax) {
+  return ((sizeof(QTab<double>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
+}
+
+hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu) {
+  if (fp64)
+    return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<true>, 64, lds_fwd64(smax))
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<false>, 64, lds_fwd64(smax));
+  return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd32<true>, 64, lds_fwd32(smax))
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd32<false>, 64, lds_fwd32(smax));
+}
+
+hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
+                           const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
+                           int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev) {
+  const int64_t nb = b.nregions * NCLS;
+  if (ev) (void)hipEventRecord(ev[0], st);
+  hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, p.ctr, (int)CTR_COUNT);
+  if (b.nreads > 0) {
+    int g = (int)((b.nreads + 255) / 256);
+    g = g > 4096 ? 4096 : g;
+    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, anchored, force64, p.read_cls, p.read_exp);
+  }
+  if (b.nregions > 0) {
+    int g = (int)(b.nregions > 65535 ? 65535 : b.nregions);
+    hipLaunchKernelGGL(k_bucket, dim3(g), dim3(256), 0, st, b, p, out);
+  }
+  hipLaunchKernelGGL(k_scan_units, dim3(1), dim3(1024), 0, st, p.unit_cnt, nb, p.ctr);
+  hipLaunchKernelGGL(k_unit_map, dim3(2048), dim3(256), 0, st, (const int64_t*)p.unit_cnt, nb,
+                     (const unsigned long long*)p.ctr, p.unit_bucket);
+  if (ev) (void)hipEventRecord(ev[1], st);
+  if (!force64 && blocks32 > 0) {
+    if (anchored)
+      hipLaunchKernelGGL(k_fwd32<true>, dim3(blocks32), dim3(64), lds_fwd32(smax), st, b, p, tab32, smax, out);
+    else
+      hipLaunchKernelGGL(k_fwd32<false>, dim3(blocks32), dim3(64), lds_fwd32(smax), st, b, p, tab32, smax, out);
+  }
+  if (ev) (void)hipEventRecord(ev[2], st);
+  if (anchored)
+    hipLaunchKernelGGL(k_fwd64<true>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
+  else
+    hipLaunchKernelGGL(k_fwd64<false>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
+  if (ev) (void)hipEventRecord(ev[3], st);
+  return hipGetLastError();
+}
+
+}  // namespace vd

@@ -181,20 +181,36 @@ def _substitute(rng, seq: np.ndarray, qual: np.ndarray) -> np.ndarray:
     return seq
 
 
+class _Reads:
+    """Reads of one region as concatenated bases / qualities plus lengths."""
+
+    def __init__(self, seq, qual, lengths):
+        self.seq, self.qual, self.lengths = seq, qual, np.asarray(lengths, dtype=np.int64)
+
+    def __len__(self):
+        return len(self.lengths)
+
+
 class _Builder:
     def __init__(self):
-        self.reads, self.quals, self.haps = [], [], []
+        self.rseq, self.rqual, self.rlen, self.haps = [], [], [], []
         self.rcount, self.hcount, self.truth = [], [], []
+        self.nhaps = 0
 
     def add(self, haps, reads, quals, truth=None):
-        base = sum(self.hcount)
+        if not isinstance(reads, _Reads):
+            reads = _Reads(np.concatenate(reads) if reads else np.zeros(0, np.uint8),
+                           np.concatenate(quals) if quals else np.zeros(0, np.uint8),
+                           [len(r) for r in reads])
+        self.rseq.append(reads.seq)
+        self.rqual.append(reads.qual)
+        self.rlen.append(reads.lengths)
         self.haps.extend(haps)
-        self.reads.extend(reads)
-        self.quals.extend(quals)
         self.rcount.append(len(reads))
         self.hcount.append(len(haps))
         if truth is not None:
-            self.truth.extend(int(t) + base for t in truth)
+            self.truth.extend(int(t) + self.nhaps for t in truth)
+        self.nhaps += len(haps)
 
     def batch(self, gap, ext) -> RegionBatch:
         def cat(arrs):
@@ -203,8 +219,11 @@ class _Builder:
             off = np.zeros(len(arrs) + 1, dtype=np.int64)
             np.cumsum([len(a) for a in arrs], out=off[1:])
             return np.concatenate(arrs).astype(np.uint8), off
-        rb, ro = cat(self.reads)
-        qb, _ = cat(self.quals)
+        rb = np.concatenate(self.rseq).astype(np.uint8) if self.rseq else np.zeros(0, np.uint8)
+        qb = np.concatenate(self.rqual).astype(np.uint8) if self.rqual else np.zeros(0, np.uint8)
+        lens = np.concatenate(self.rlen) if self.rlen else np.zeros(0, np.int64)
+        ro = np.zeros(len(lens) + 1, dtype=np.int64)
+        np.cumsum(lens, out=ro[1:])
         hb, ho = cat(self.haps)
         rr = np.zeros(len(self.rcount) + 1, dtype=np.int64)
         rh = np.zeros(len(self.hcount) + 1, dtype=np.int64)
@@ -221,17 +240,25 @@ def _region_haps(rng, length, count, snv_max=5, indel_max=2, max_indel=10):
 
 
 def _sample_reads(rng, haps, lengths, qual_fn):
-    reads, quals, truth = [], [], []
-    for n in lengths:
-        h = int(rng.integers(0, len(haps)))
-        hap = haps[h]
-        n = int(min(n, len(hap)))
-        o = int(rng.integers(0, len(hap) - n + 1))
-        q = qual_fn(n)
-        reads.append(_substitute(rng, hap[o:o + n], q))
-        quals.append(q)
-        truth.append(h)
-    return reads, quals, truth
+    """Reads of the given lengths sampled from random haplotypes at random offsets,
+    with qualities qual_fn(total) and substitution errors (vectorised per region)."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    if len(lengths) == 0:
+        return [], [], []
+    hcat = np.concatenate(haps)
+    hoff = np.zeros(len(haps) + 1, dtype=np.int64)
+    np.cumsum([len(h) for h in haps], out=hoff[1:])
+    hl = np.diff(hoff)
+    src = rng.integers(0, len(haps), len(lengths))
+    n = np.minimum(lengths, hl[src])
+    o = (rng.random(len(n)) * (hl[src] - n + 1)).astype(np.int64)
+    starts = hoff[src] + o
+    roff = np.zeros(len(n) + 1, dtype=np.int64)
+    np.cumsum(n, out=roff[1:])
+    idx = np.repeat(starts - roff[:-1], n) + np.arange(roff[-1])
+    q = qual_fn(int(roff[-1]))
+    seq = _substitute(rng, hcat[idx], q)
+    return _Reads(seq, q, n), None, src.tolist()
 
 
 def single_region(seed: int = 41001, reads: int = 64, haps: int = 8, read_len: int = 150,
