@@ -437,7 +437,6 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   const int pad = ntiles * GROUP * R - np;
   const int64_t rb = b.read_off[gp.read];
   const double gam = tab->gamma, xx = tab->ext;
-  const double gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : 0.0) : gam;
   const double C = 0x1p500;
   // boundary buffers: [buffer][stream][group][slot], slot = column + 17
   double* bufP[2];
@@ -454,6 +453,9 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   __syncthreads();
   for (int tile = 0; tile < ntiles; ++tile) {
     const bool first = tile == 0, last = tile == ntiles - 1;
+    // the read's last row (lane 15, register R-1 of the last tile) carries the
+    // terminal weights; in earlier tiles that register is an interior row
+    const double gam_last = (l == GROUP - 1 && last) ? (ANCH ? tab->gap : 0.0) : gam;
     double gd[R], bd[R], mu[R], be[R];
     uint32_t rc[R];
 #pragma unroll
