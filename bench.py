@@ -66,6 +66,19 @@ def workload(name: str, regions: int):
     raise ValueError(name)
 
 
+def measured_traffic(args):
+    """HBM bytes per k_fwd32 launch from the committed PMC summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) when it was measured on this workload, else None."""
+    f = ROOT / "profiles" / "k_fwd32_uneven_regions_4096_traffic.json"
+    if args.workload != "uneven" or args.regions != 4096 or args.mode != 0 or args.precision != "mixed":
+        return None
+    try:
+        t = json.loads(f.read_text())
+        return int(t["fetch_bytes"] + t["write_bytes"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def fp32_cells(batch, mode: int) -> int:
     """Cells of the pairs the FP32 kernel processes (reads of <= 256 rows)."""
     rl = batch.read_lengths() + (1 if mode == 1 else 0)
@@ -156,6 +169,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "float64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the regions shard across GPUs with no exchange, "
@@ -192,6 +207,13 @@ def main():
         ms_step, last["plan_ms"], last["fwd32_ms"], last["fwd64_ms"], last["flagged"], last["fp64_pairs"]),
         file=sys.stderr, flush=True)
 
+    if args.copy_calib:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     cpu = None
     if not args.no_cpu:
         threads = min(available_cpus(), 16)
@@ -210,7 +232,8 @@ def main():
                                 "fwd64": round(last["fwd64_ms"], 3)}},
         "roofline": {"bound": "valu", "kernel": "k_fwd32", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": None, "flops_per_cell": FLOPS_PER_CELL, "cells_per_launch": cells32,
+                     "traffic": measured_traffic(args), "traffic_unit": "bytes/launch (rocprofv3 PMC)",
+                     "flops_per_cell": FLOPS_PER_CELL, "cells_per_launch": cells32,
                      "kernel_ms": round(fwd32, 3)},
         "cpu_baseline": cpu,
     }
