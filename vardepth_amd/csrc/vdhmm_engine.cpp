@@ -156,6 +156,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   if (int rc = upload_params(*params, st)) return rc;
   const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
   const int force64 = precision == VD_PREC_FP64 ? 1 : 0;
+  // hap-code / boundary slots per group: columns -17 .. m+16
   const int smax = ((max_hap_len + 48) + 7) & ~7;
   // plan buffers
   const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
@@ -177,7 +178,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     g.occ_smax = smax;
   }
   const int blocks32 = std::max(1, g.cus * std::max(1, g.occ32[anchored]));
-  const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 4)));
+  const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   vd::BatchDev b{db->read_bytes, db->qual_bytes, db->read_off, db->hap_bytes, db->hap_off,
                  db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};

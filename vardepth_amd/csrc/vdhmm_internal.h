@@ -17,16 +17,20 @@ constexpr int MAX_R32 = 16;
 
 constexpr int GROUP = 16;        // lanes per pair (one DPP row)
 constexpr int PAIRS_PER_UNIT = 4;
-constexpr int R64 = 4;           // FP64 rows per lane -> 64-row tiles
+constexpr int R64 = 4;           // FP64 rows per lane: 16 lanes x 4 = 64-row tiles
 
 // hap code stream sentinels (hap bytes are 0..255)
 constexpr uint32_t SENT_POST = 0x100u;  // column beyond the haplotype end
 constexpr uint32_t SENT_PRE = 0x101u;   // column <= 0
 constexpr uint32_t NOMATCH = 0xFFFFu;   // read code of dummy rows
 
-constexpr int CEXP32 = 96;    // FP32 start scale 2^96
+// FP32 start scale 2^108: every scaled value stays below 2^108 * 1.2 * (m+1),
+// so haplotypes up to 2^18 columns cannot overflow.  A pair whose scaled
+// terminal is below 2^-100 leaves the FP32 window (its dominant paths would
+// pass within 2^26 of the denormal range) and goes to the FP64 rescue.
+constexpr int CEXP32 = 108;
 constexpr int CEXP64 = 500;   // FP64 start / tile-renormalised scale 2^500
-constexpr float FLAG32_MIN = 0x1p-90f;  // FP32 terminal below this -> FP64 rescue
+constexpr float FLAG32_MIN = 0x1p-100f;
 
 // Per-quality-bin row constants of the rescaled recurrence, for one
 // precision.  With a = 1-2g, b = 1-x, c = g/4, d = x/4 and a per-bin

@@ -26,8 +26,8 @@
 //   Anchored mode prepends a virtual row 0 (e'=1, mu=beta=0) that builds the
 //   reference's init row M(0,0)=1, D(0,j)=g x^(j-1).
 //
-// FP32 runs the bulk with a start scale of 2^96; a pair whose scaled
-// terminal falls below 2^-90 is appended to the FP64 work list.  The FP64
+// FP32 runs the bulk with a start scale of 2^108; a pair whose scaled
+// terminal falls below 2^-100 is appended to the FP64 work list.  The FP64
 // kernel uses the same recurrence with 64-row tiles, renormalising the
 // boundary row between tiles, so it covers any read length.
 #include "vdhmm_internal.h"
@@ -35,6 +35,7 @@
 namespace vd {
 
 #define LN2 0.69314718055994530942
+#define LN_1EM300 (-690.77552789821368)  // ln(1e-300): the reference's terminal clamp
 
 __device__ __forceinline__ float dpp_shr1(float old, float src) {
   return __builtin_bit_cast(
@@ -60,22 +61,29 @@ __device__ __forceinline__ double fmaT(double a, double b, double c) { return __
 // ---------------------------------------------------------------------------
 // planning
 
-// Class and exponent sum of every read.
+// Class and exponent sum of every read: one wave per read, coalesced loads
+// of the quality bytes, wave reduction of the per-bin exponents.
 __global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, int anchored, int force64,
                              int8_t* __restrict__ cls, int32_t* __restrict__ exps) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < b.nreads;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    int64_t o0 = b.read_off[r], o1 = b.read_off[r + 1];
-    int n = (int)(o1 - o0);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = wave0; r < b.nreads; r += nwaves) {
+    const int64_t o0 = b.read_off[r], o1 = b.read_off[r + 1];
     int e = 0;
-    for (int64_t k = o0; k < o1; ++k) {
+    for (int64_t k = o0 + lane; k < o1; k += 64) {
       int q = b.qual_bytes[k];
       e += qE[q < 63 ? q : 63];
     }
-    int np = n + anchored;
-    int c = n == 0 ? CLS_EMPTY : ((force64 || np > GROUP * MAX_R32) ? CLS_F64 : (np + GROUP - 1) / GROUP);
-    cls[r] = (int8_t)c;
-    exps[r] = e;
+#pragma unroll
+    for (int off = 32; off; off >>= 1) e += __shfl_xor(e, off);
+    if (lane == 0) {
+      const int n = (int)(o1 - o0);
+      const int np = n + anchored;
+      const int c = n == 0 ? CLS_EMPTY : ((force64 || np > GROUP * MAX_R32) ? CLS_F64 : (np + GROUP - 1) / GROUP);
+      cls[r] = (int8_t)c;
+      exps[r] = e;
+    }
   }
 }
 
@@ -279,7 +287,7 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
   float M[R], D[R], Q[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.f;
-  const float C = 0x1p96f;
+  const float C = 0x1p108f;
   float Plast = 0.f, Xprev = 0.f, acc = 0.f, rmax = 0.f, latch = 0.f;
   const uint16_t* cl = codes + g * smax + 16 - l;
   // step 0 (column -1 for lane 0): injection of the boundary row's first values
@@ -333,9 +341,11 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
     double v;
     bool flag;
     if (ANCH) {
+      // reference clamp ln(max(1e-300 * rowmax, L)), evaluated in log space
       flag = !(latch >= FLAG32_MIN) || !(rmax < __builtin_inff());
-      double cl2 = 1e-300 * (double)rmax;
-      v = log((double)latch > cl2 ? (double)latch : cl2) + lnk;
+      const double lc = log((double)rmax) + LN_1EM300;
+      const double ll = log((double)latch);
+      v = (ll > lc ? ll : lc) + lnk;
     } else {
       flag = !(acc >= FLAG32_MIN) || !(acc < __builtin_inff());
       v = log((double)acc) + lnk - log((double)(gp.m + 1));
@@ -415,10 +425,11 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
 
 
 // ---------------------------------------------------------------------------
-// FP64: 64-row tiles (R64 rows per lane).  Tile boundaries go through a global
-// workspace (lane 15 writes the last row's P/Q streams, lane 0 of the next
-// tile injects them), renormalised to 2^CEXP64 so that any read length stays
-// inside the FP64 range.
+// FP64: the FP32 layout (4 pairs per wave, 16-lane groups, DPP row_shr:1) with
+// R64 = 4 rows per lane, i.e. 64-row tiles.  A tile's boundary row (the last
+// row's P/Q streams) goes through a global workspace and is renormalised to
+// 2^CEXP64 before the next tile, so any read length stays in range: a tile
+// decays by at most 64 ln(4/x) nats along the insertion path.
 
 template <bool ANCH>
 __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, const QTab<double>* __restrict__ tab,
@@ -428,8 +439,7 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
   const int tend = gp.m_max + 16;
   const int tmain = gp.m_min + 2;
-  const int nslots = tend + 18;
-  fill_codes(b, gp, codes, smax, nslots);
+  fill_codes(b, gp, codes, smax, tend + 18);
   const int np = gp.n + (ANCH ? 1 : 0);
   int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
   int ntiles = max(ntile_g, __shfl_xor(ntile_g, 16));
@@ -437,7 +447,6 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   const int pad = ntiles * GROUP * R - np;
   const int64_t rb = b.read_off[gp.read];
   const double gam = tab->gamma, xx = tab->ext;
-  const double C = 0x1p500;
   // boundary buffers: [buffer][stream][group][slot], slot = column + 17
   double* bufP[2];
   double* bufQ[2];
@@ -447,7 +456,7 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     bufQ[k] = ws + ((size_t)(2 * k + 1) * PAIRS_PER_UNIT + g) * smax;
   }
   int shift_total = 0;   // sum of tile renormalisation exponents (log2)
-  double sc = 1.0;       // scale applied to the injected boundary of this tile
+  double sc1 = 1.0, sc2 = 1.0;  // 2^s of this tile's injected boundary, as two factors
   double acc = 0.0, rmax = 0.0, latch = 0.0;
   const uint16_t* cl = codes + g * smax + 16 - l;
   __syncthreads();
@@ -490,6 +499,7 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     const double* inQ = bufQ[(tile + 1) & 1];
     double* oP = bufP[tile & 1];
     double* oQ = bufQ[tile & 1];
+    const double C = 0x1p500;
     for (int t = 0; t <= tend; ++t) {
       const uint32_t code = cl[t];
       double pinj, qinj;
@@ -503,8 +513,8 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
       } else {
         // column t-1 of the previous tile's last row (slot t+16); none past m_max
         const bool have = t - 1 <= gp.m_max;
-        pinj = have ? inP[t + 16] * sc : 0.0;
-        qinj = have ? inQ[t + 16] * sc : 0.0;
+        pinj = have ? inP[t + 16] * sc1 * sc2 : 0.0;
+        qinj = have ? inQ[t + 16] * sc1 * sc2 : 0.0;
       }
       const double Pin = dpp_shr1(pinj, Plast);
       const double Xn = dpp_shr1(qinj, Q[R - 1]);
@@ -521,20 +531,23 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
           acc += Plast;
         }
       } else if (l == GROUP - 1) {
-        const int slot = t + 1;  // column t-16
-        oP[slot] = Plast;
-        oQ[slot] = Q[R - 1];
+        oP[t + 1] = Plast;  // column t-16 -> slot t+1
+        oQ[t + 1] = Q[R - 1];
         bmax = fmax(bmax, fmax(Plast, Q[R - 1]));
       }
     }
     if (!last) {
-      // renormalise the boundary row to 2^CEXP64
+      // renormalise the boundary row to 2^CEXP64; 2^s is applied as two
+      // factors because s can exceed the FP64 exponent range of one factor
       int e = 0;
       if (bmax > 0.0) frexp(bmax, &e);
-      const int s = bmax > 0.0 ? CEXP64 - e : 0;
-      const double scl = ldexp(1.0, s);
-      sc = __shfl(scl, (g * GROUP) + GROUP - 1);
-      shift_total -= __shfl(s, (g * GROUP) + GROUP - 1);
+      int s = bmax > 0.0 ? CEXP64 - e : 0;
+      s = s > 2040 ? 2040 : s;
+      const int src = g * GROUP + GROUP - 1;
+      const int s1 = s / 2, s2 = s - s1;
+      sc1 = __shfl(ldexp(1.0, s1), src);
+      sc2 = __shfl(ldexp(1.0, s2), src);
+      shift_total -= __shfl(s, src);
       __threadfence();
       __syncthreads();
     }
@@ -544,9 +557,11 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     double v;
     bool bad;
     if (ANCH) {
-      const double cl2 = 1e-300 * rmax;
+      // reference clamp ln(max(1e-300 * rowmax, L)), evaluated in log space
       bad = !(rmax > 0.0) || !(rmax < __builtin_inf());
-      v = log(latch > cl2 ? latch : cl2) + lnk;
+      const double lc = log(rmax) + LN_1EM300;
+      const double ll = log(latch);
+      v = (ll > lc ? ll : lc) + lnk;
     } else {
       bad = !(acc > 0.0) || !(acc < __builtin_inf());
       v = log(acc) + lnk - log((double)(gp.m + 1));
@@ -644,8 +659,8 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
   if (ev) (void)hipEventRecord(ev[0], st);
   hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, p.ctr, (int)CTR_COUNT);
   if (b.nreads > 0) {
-    int g = (int)((b.nreads + 255) / 256);
-    g = g > 4096 ? 4096 : g;
+    int g = (int)((b.nreads + 3) / 4);  // 4 waves of 64 per block, one read per wave
+    g = g > 8192 ? 8192 : g;
     hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, anchored, force64, p.read_cls, p.read_exp);
   }
   if (b.nregions > 0) {
@@ -663,7 +678,7 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
       hipLaunchKernelGGL(k_fwd32<false>, dim3(blocks32), dim3(64), lds_fwd32(smax), st, b, p, tab32, smax, out);
   }
   if (ev) (void)hipEventRecord(ev[2], st);
-  if (anchored)
+if (anchored)
     hipLaunchKernelGGL(k_fwd64<true>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
   else
     hipLaunchKernelGGL(k_fwd64<false>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
