@@ -134,6 +134,27 @@ int vd_forward_device(const vd_batch *dev_batch, const vd_params *params, int mo
                       int64_t npairs, int max_hap_len, double *dev_out, void *stream,
                       vd_stats *stats);
 
+/* ---- gradients (no counterpart in the reference) ------------------------ */
+
+/* Gradient of sum_p w_p ln L_p over every pair of the batch with respect to
+ * the model parameters, computed by the adjoint kernels (FP32 with FP64 for
+ * the pairs the FP32 pass flags).  grad[0] = d/dgap, grad[1] = d/dext,
+ * grad[2 + q] = d/derr[q] (q = 0..63).  weights: one per output slot, NULL =
+ * all 1.  out (optional): the ln likelihoods, as vd_forward(mixed) returns.
+ * Mode 2 is differentiated as mode 0 (same model).  Reads must be at most 256
+ * bases (255 anchored).  The error-table bins enter through the rescaled
+ * emissions with each bin's power-of-two normaliser held fixed, so a gradient
+ * is exact away from the error values where that normaliser changes.
+ * Host pointers; returns 0, or 2 with vd_last_error() set. */
+int vd_grad(const vd_batch *batch, const vd_params *params, int mode, const double *weights, double *out,
+            double *grad, vd_stats *stats);
+
+/* Same with every buffer already in HBM, enqueued on `stream`; max_read_len
+ * bounds every read length.  dev_grad receives the 66 doubles. */
+int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode, int64_t npairs,
+                   int max_hap_len, int max_read_len, const double *dev_weights, double *dev_out,
+                   double *dev_grad, void *stream, vd_stats *stats);
+
 /* Message of the last failed call on this process. */
 const char *vd_last_error(void);
 /* Device index used by the library (VARDEPTH_CUDA_DEVICE, default 0). */

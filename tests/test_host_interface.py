@@ -246,3 +246,15 @@ def test_from_regions_layout():
                                 (["T"], [b"\x09"], ["TT"])])
     assert b.npairs == 7 and b.cells() == 2 * 4 + 3 * 4 + 2
     assert b.out_off().tolist() == [0, 6, 7]
+
+
+def test_grad_rejects_long_reads_before_device():
+    from vardepth_amd import _lib
+    b = workloads.from_regions([(["A" * 300], [b"\x1e" * 300], ["A" * 310])])
+    with pytest.raises(_lib.VdhmmError, match="256"):
+        engine.grad(b)
+    b1 = workloads.from_regions([(["A" * 256], [b"\x1e" * 256], ["A" * 260])])
+    with pytest.raises(_lib.VdhmmError, match="255 in anchored"):
+        engine.grad(b1, mode=1)
+    with pytest.raises(ValueError, match="one entry per"):
+        engine.grad(b1, weights=[1.0, 2.0])

@@ -48,7 +48,8 @@ def build(verbose: bool = False) -> Path:
     objs = []
     tmp = PKG / "_build"
     tmp.mkdir(exist_ok=True)
-    for src, lang in ((CSRC / "vdhmm_kernels.hip", "hip"), (CSRC / "vdhmm_engine.cpp", "c++")):
+    for src, lang in ((CSRC / "vdhmm_kernels.hip", "hip"), (CSRC / "vdhmm_grad.hip", "hip"),
+                      (CSRC / "vdhmm_engine.cpp", "c++")):
         obj = tmp / (src.stem + ".o")
         cmd = [hc] + HIP_FLAGS + (["-x", "hip"] if lang == "hip" else []) + ["-c", str(src), "-o", str(obj)]
         if verbose:

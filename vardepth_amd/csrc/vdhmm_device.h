@@ -1,0 +1,129 @@
+// Device helpers shared by the forward kernels (vdhmm_kernels.hip) and the
+// adjoint kernels (vdhmm_grad.hip).
+#pragma once
+#include "vdhmm_internal.h"
+
+namespace vd {
+
+#define LN2 0.69314718055994530942
+#define LN_1EM300 (-690.77552789821368)  // ln(1e-300): the reference's terminal clamp
+
+__device__ __forceinline__ float dpp_shr1(float old, float src) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
+                                         0x111, 0xF, 0xF, false));
+}
+
+struct I2 {
+  int lo, hi;
+};
+
+__device__ __forceinline__ double dpp_shr1(double old, double src) {
+  I2 o = __builtin_bit_cast(I2, old), s = __builtin_bit_cast(I2, src);
+  I2 r;
+  r.lo = __builtin_amdgcn_update_dpp(o.lo, s.lo, 0x111, 0xF, 0xF, false);
+  r.hi = __builtin_amdgcn_update_dpp(o.hi, s.hi, 0x111, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// ---------------------------------------------------------------------------
+// the rescaled anti-diagonal step for R rows of one lane
+
+template <typename T, int R, bool TAIL>
+__device__ __forceinline__ T adiag_step(T (&M)[R], T (&D)[R], T (&Q)[R], const T (&gd)[R], const T (&bd)[R],
+                                        const T (&mu)[R], const T (&be)[R], const uint32_t (&rc)[R], T gam,
+                                        T gam_last, T xx, T Pin, T Qin, uint32_t code) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) D[r] = fmaT(xx, D[r], M[r]);
+  const bool post = TAIL && code == SENT_POST;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    T qup = r == 0 ? Qin : Q[r - 1];
+    T e = rc[r] == code ? gd[r] : bd[r];
+    if (TAIL) e = post ? T(0) : e;
+    M[r] = e * qup;
+  }
+  T Ih = Pin;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T P = fmaT(mu[r], Ih, M[r]);
+    Q[r] = fmaT(r == R - 1 ? gam_last : gam, D[r], fmaT(be[r], Ih, M[r]));
+    Ih = P;
+  }
+  return Ih;
+}
+
+// Per-lane description of the lane's group pair inside a unit.
+struct GroupPair {
+  int64_t read, hap, out;
+  int n, m;        // read / haplotype length (m = unit max for invalid groups)
+  int valid;
+  int m_min, m_max;  // over the unit's valid groups
+};
+
+// Fill the unit's hap-code slots: slot k of group g holds the code of column
+// k-17 (SENT_PRE for columns <= 0, SENT_POST past the end).
+__device__ __forceinline__ void fill_codes(const BatchDev& b, const GroupPair& gp, uint16_t* codes, int smax,
+                                           int nslots) {
+  const int lane = threadIdx.x & 63;
+  int64_t hb[PAIRS_PER_UNIT];
+  int hm[PAIRS_PER_UNIT];
+#pragma unroll
+  for (int g = 0; g < PAIRS_PER_UNIT; ++g) {
+    int64_t hoff = b.hap_off[gp.hap];
+    hb[g] = __shfl(hoff, g * GROUP);
+    hm[g] = __shfl(gp.valid ? gp.m : 0, g * GROUP);
+  }
+  for (int kk = lane; kk < nslots; kk += 64) {
+    const int col = kk - 17;
+#pragma unroll
+    for (int g = 0; g < PAIRS_PER_UNIT; ++g) {
+      uint32_t c = col <= 0 ? SENT_PRE : (col > hm[g] ? SENT_POST : (uint32_t)b.hap_bytes[hb[g] + col - 1]);
+      codes[g * smax + kk] = (uint16_t)c;
+    }
+  }
+}
+
+
+// FP32 unit u -> the lane's group pair; returns the unit's class (rows per
+// lane), or -1 when every group of the unit is empty.  Pairs with an empty
+// haplotype get -inf here and are marked invalid.
+__device__ __forceinline__ int decode_unit32(const BatchDev& b, const PlanDev& p, unsigned long long u,
+                                             GroupPair& gp, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const int bkt = p.unit_bucket[u];
+  const int64_t j = (int64_t)u - p.unit_cnt[bkt];
+  const int64_t k = bkt / NCLS;
+  const int c = bkt % NCLS;
+  const int64_t r0 = b.region_reads[k], h0 = b.region_haps[k];
+  const int64_t H = b.region_haps[k + 1] - h0;
+  const int32_t* co = p.class_off + k * (NCLS + 1);
+  const int64_t cnt = co[c + 1] - co[c];
+  const int64_t pi = j * PAIRS_PER_UNIT + g;
+  gp.valid = pi < cnt * H;
+  const int64_t s = gp.valid ? pi / H : 0, h = gp.valid ? pi % H : 0;
+  gp.read = p.read_perm[r0 + co[c] + s];
+  gp.hap = h0 + h;
+  gp.out = b.out_off[k] + (gp.read - r0) * H + h;
+  gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+  gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+  if (gp.valid && gp.m == 0) {
+    if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
+    gp.valid = 0;
+  }
+  int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
+  mx = max(mx, __shfl_xor(mx, 16));
+  mx = max(mx, __shfl_xor(mx, 32));
+  mn = min(mn, __shfl_xor(mn, 16));
+  mn = min(mn, __shfl_xor(mn, 32));
+  if (mx == 0) return -1;
+  gp.m_max = mx;
+  gp.m_min = mn;
+  if (!gp.valid) gp.m = mx;
+  return c;
+}
+
+}  // namespace vd

@@ -67,11 +67,15 @@ struct Engine {
   // plan scratch
   DevBuf read_cls, read_exp, class_off, read_perm, unit_cnt, unit_bucket, f64_list, ctr, ws64;
   // parameter tables
-  DevBuf qE, tab32, tab64;
+  DevBuf qE, tab32, tab64, gtab;
+  // gradient scratch: checkpoints, per-block partial gradients, staged weights / gradient
+  DevBuf ck32, ck64, gpart, in_w, in_grad;
   vd_params params_on_device;
   bool have_params = false;
   int occ32[2] = {0, 0}, occ64[2] = {0, 0};
   int occ_smax = -1;
+  int gocc32[2] = {0, 0}, gocc64[2] = {0, 0};
+  int gocc_smax = -1;
 };
 
 Engine g;
@@ -145,6 +149,32 @@ int upload_params(const vd_params& p, hipStream_t st) {
   return 0;
 }
 
+int plan_buffers(const vd_batch* db, int64_t npairs) {
+  const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
+  const int64_t nb = db->nregions * vd::NCLS;
+  const int64_t max_units = npairs / vd::PAIRS_PER_UNIT + nb + 1;
+  HIPCHK(g.read_cls.ensure(nreads));
+  HIPCHK(g.read_exp.ensure(nreads * 4));
+  HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
+  HIPCHK(g.read_perm.ensure(nreads * 8));
+  HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
+  HIPCHK(g.unit_bucket.ensure(max_units * 4));
+  HIPCHK(g.f64_list.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
+  HIPCHK(g.ctr.ensure(vd::CTR_COUNT * 8));
+  return 0;
+}
+
+vd::BatchDev batch_dev(const vd_batch* db) {
+  return vd::BatchDev{db->read_bytes, db->qual_bytes, db->read_off, db->hap_bytes, db->hap_off,
+                      db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};
+}
+
+vd::PlanDev plan_dev(int64_t npairs) {
+  return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
+                     g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
+                     g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1)};
+}
+
 int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
                           int max_hap_len, double* dout, hipStream_t st, vd_stats* stats) {
   if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
@@ -158,18 +188,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   const int force64 = precision == VD_PREC_FP64 ? 1 : 0;
   // hap-code / boundary slots per group: columns -17 .. m+16
   const int smax = ((max_hap_len + 48) + 7) & ~7;
-  // plan buffers
-  const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
-  const int64_t nb = db->nregions * vd::NCLS;
-  const int64_t max_units = npairs / vd::PAIRS_PER_UNIT + nb + 1;
-  HIPCHK(g.read_cls.ensure(nreads));
-  HIPCHK(g.read_exp.ensure(nreads * 4));
-  HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
-  HIPCHK(g.read_perm.ensure(nreads * 8));
-  HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
-  HIPCHK(g.unit_bucket.ensure(max_units * 4));
-  HIPCHK(g.f64_list.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
-  HIPCHK(g.ctr.ensure(vd::CTR_COUNT * 8));
+  if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.occ_smax != smax) {
     for (int a = 0; a < 2; ++a) {
       HIPCHK(vd::occupancy(a, 0, smax, &g.occ32[a]));
@@ -180,11 +199,8 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   const int blocks32 = std::max(1, g.cus * std::max(1, g.occ32[anchored]));
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
-  vd::BatchDev b{db->read_bytes, db->qual_bytes, db->read_off, db->hap_bytes, db->hap_off,
-                 db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};
-  vd::PlanDev p{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
-                g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
-                g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1)};
+  const vd::BatchDev b = batch_dev(db);
+  const vd::PlanDev p = plan_dev(npairs);
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks64,
                              g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr));
@@ -243,15 +259,7 @@ int validate_host_batch(const vd_batch* b, int64_t* npairs, int* max_hap, uint64
   return 0;
 }
 
-int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, int precision, double* out,
-                        vd_stats* stats) {
-  auto t0 = std::chrono::steady_clock::now();
-  int64_t npairs = 0;
-  int max_hap = 0;
-  uint64_t cells = 0;
-  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
-  if (int rc = engine_init()) return rc;
-  hipStream_t st = g.own_stream;
+int stage_host_batch(const vd_batch* hb, int64_t npairs, hipStream_t st) {
   const int64_t nrb = hb->read_off[hb->nreads], nhb = hb->hap_off[hb->nhaps];
   HIPCHK(g.in_read.ensure(nrb + 1));
   HIPCHK(g.in_qual.ensure(nrb + 1));
@@ -272,9 +280,26 @@ int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, i
   HIPCHK(hipMemcpyAsync(g.in_rreg.p, hb->region_reads, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(g.in_hreg.p, hb->region_haps, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(g.in_ooff.p, hb->out_off, (hb->nregions + 1) * 8, hipMemcpyHostToDevice, st));
-  vd_batch db{g.in_read.as<uint8_t>(), g.in_qual.as<uint8_t>(), g.in_roff.as<int64_t>(), g.in_hap.as<uint8_t>(),
-              g.in_hoff.as<int64_t>(), g.in_rreg.as<int64_t>(), g.in_hreg.as<int64_t>(),
-              g.in_ooff.as<int64_t>(), hb->nregions, hb->nreads, hb->nhaps};
+  return 0;
+}
+
+vd_batch staged_view(const vd_batch* hb) {
+  return vd_batch{g.in_read.as<uint8_t>(), g.in_qual.as<uint8_t>(), g.in_roff.as<int64_t>(), g.in_hap.as<uint8_t>(),
+                  g.in_hoff.as<int64_t>(), g.in_rreg.as<int64_t>(), g.in_hreg.as<int64_t>(),
+                  g.in_ooff.as<int64_t>(), hb->nregions, hb->nreads, hb->nhaps};
+}
+
+int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, int precision, double* out,
+                        vd_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  int64_t npairs = 0;
+  int max_hap = 0;
+  uint64_t cells = 0;
+  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  if (int rc = stage_host_batch(hb, npairs, st)) return rc;
+  vd_batch db = staged_view(hb);
   vd_stats local;
   vd_stats* s = stats ? stats : &local;
   std::memset(s, 0, sizeof(*s));
@@ -285,6 +310,129 @@ int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, i
   s->cells = cells;
   s->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (s->fp64_bad) return fail("vdhmm: FP64 pass produced non-finite likelihoods");
+  return 0;
+}
+
+// ---- gradients -------------------------------------------------------------
+
+void build_grad_tab(const vd_params& p, vd::GradTab& t) {
+  const double g = p.gap, x = p.ext, a = 1 - 2 * g;
+  for (int q = 0; q < 64; ++q) {
+    const double e = p.err[q];
+    const double best = std::max(1 - e, e / 3) * a;
+    int ex = 0;
+    if (best > 0) std::frexp(best, &ex);
+    const double rho = std::ldexp(1.0, ex);
+    t.dgood_dg[q] = -2 * (1 - e) / rho;
+    t.dbad_dg[q] = -2 * (e / 3) / rho;
+    t.dgood_de[q] = -a / rho;
+    t.dbad_de[q] = a / (3 * rho);
+    t.dmu_dx[q] = 0.25 / rho;
+    t.dbeta_dx[q] = -g / (4 * a * rho);
+    t.dbeta_dg[q] = (1 - x) / (4 * rho * a * a);
+    t.dcfin_dg[q] = 0.25 / rho;
+  }
+  t.dgam_dg = (1 - x) / (a * a);
+  t.dgam_dx = -g / a;
+}
+
+int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, int64_t npairs, int max_hap_len,
+                       int max_read_len, const double* dweights, double* dout, double* dgrad, hipStream_t st,
+                       vd_stats* stats) {
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (db->nregions < 0 || db->nreads < 0 || db->nhaps < 0 || npairs < 0 || max_hap_len < 0 || max_read_len < 0)
+    return fail("vdhmm: negative batch size");
+  const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
+  if (max_read_len + anchored > vd::GROUP * vd::MAX_R32)
+    return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  if (int rc = upload_params(*params, st)) return rc;
+  static vd::GradTab gt;
+  build_grad_tab(*params, gt);
+  HIPCHK(g.gtab.ensure(sizeof(gt)));
+  HIPCHK(hipMemcpyAsync(g.gtab.p, &gt, sizeof(gt), hipMemcpyHostToDevice, st));
+  const int smax = ((max_hap_len + 48) + 7) & ~7;
+  if (int rc = plan_buffers(db, npairs)) return rc;
+  if (g.gocc_smax != smax) {
+    for (int a = 0; a < 2; ++a) {
+      HIPCHK(vd::grad_occupancy(a, 0, smax, &g.gocc32[a]));
+      HIPCHK(vd::grad_occupancy(a, 1, smax, &g.gocc64[a]));
+    }
+    g.gocc_smax = smax;
+  }
+  const int blocks32 = std::max(1, g.cus * std::max(1, g.gocc32[anchored]));
+  const int blocks64 = std::max(1, std::min(g.cus, g.cus * std::max(1, g.gocc64[anchored])));
+  HIPCHK(g.ck32.ensure((size_t)blocks32 * vd::grad_ck_stride(smax, 0) * sizeof(float)));
+  HIPCHK(g.ck64.ensure((size_t)blocks64 * vd::grad_ck_stride(smax, 1) * sizeof(double)));
+  HIPCHK(g.gpart.ensure((size_t)(blocks32 + blocks64) * vd::NGRAD * sizeof(double)));
+  const vd::BatchDev b = batch_dev(db);
+  const vd::PlanDev p = plan_dev(npairs);
+  HIPCHK(vd::launch_grad(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(), g.tab64.as<vd::QTab<double>>(),
+                         g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, blocks64, g.ck32.as<float>(),
+                         g.ck64.as<double>(), dweights, dout, g.gpart.as<double>(), dgrad, st,
+                         stats ? g.ev : nullptr));
+  if (stats) {
+    HIPCHK(hipEventSynchronize(g.ev[3]));
+    unsigned long long c[vd::CTR_COUNT];
+    HIPCHK(hipMemcpy(c, g.ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+    float ms = 0, a = 0, b2 = 0, c3 = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g.ev[0], g.ev[3]));
+    HIPCHK(hipEventElapsedTime(&a, g.ev[0], g.ev[1]));
+    HIPCHK(hipEventElapsedTime(&b2, g.ev[1], g.ev[2]));
+    HIPCHK(hipEventElapsedTime(&c3, g.ev[2], g.ev[3]));
+    stats->plan_ms = a;
+    stats->fwd32_ms = b2;
+    stats->fwd64_ms = c3;
+    stats->pairs = (uint64_t)npairs;
+    stats->fp32_units = c[vd::CTR_NUNITS];
+    stats->flagged = c[vd::CTR_FLAGGED];
+    stats->fp64_pairs = std::min<unsigned long long>(c[vd::CTR_F64], (unsigned long long)std::max<int64_t>(npairs, 1));
+    stats->fp64_bad = c[vd::CTR_F64BAD];
+    stats->overflow = c[vd::CTR_OVERFLOW];
+    stats->device_ms = ms;
+    if (c[vd::CTR_OVERFLOW]) return fail("vdhmm: FP64 work list overflow");
+  }
+  return 0;
+}
+
+int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, const double* weights, double* out,
+                     double* grad, vd_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  int64_t npairs = 0;
+  int max_hap = 0;
+  uint64_t cells = 0;
+  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  int max_read = 0;
+  for (int64_t i = 0; i < hb->nreads; ++i)
+    max_read = std::max<int>(max_read, (int)std::min<int64_t>(hb->read_off[i + 1] - hb->read_off[i], 1 << 30));
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (max_read + (mode == VD_MODE_ANCHORED ? 1 : 0) > vd::GROUP * vd::MAX_R32)
+    return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  if (int rc = stage_host_batch(hb, npairs, st)) return rc;
+  HIPCHK(g.in_grad.ensure(vd::NGRAD * sizeof(double)));
+  const double* dw = nullptr;
+  if (weights) {
+    HIPCHK(g.in_w.ensure(std::max<int64_t>(npairs, 1) * 8));
+    if (npairs) HIPCHK(hipMemcpyAsync(g.in_w.p, weights, npairs * 8, hipMemcpyHostToDevice, st));
+    dw = g.in_w.as<double>();
+  }
+  vd_batch db = staged_view(hb);
+  vd_stats local;
+  vd_stats* s = stats ? stats : &local;
+  std::memset(s, 0, sizeof(*s));
+  if (int rc = grad_device_locked(&db, params, mode, npairs, max_hap, max_read, dw, g.in_out.as<double>(),
+                                  g.in_grad.as<double>(), st, s))
+    return rc;
+  if (out && npairs) HIPCHK(hipMemcpyAsync(out, g.in_out.p, npairs * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(grad, g.in_grad.p, vd::NGRAD * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s->cells = cells;
+  s->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (s->fp64_bad) return fail("vdhmm: FP64 gradient pass met non-finite likelihoods");
   return 0;
 }
 
@@ -427,6 +575,28 @@ int vd_forward_device(const vd_batch* dev_batch, const vd_params* params, int mo
   if (stats) std::memset(stats, 0, sizeof(*stats));
   int rc = forward_device_locked(dev_batch, params, mode, precision, npairs, max_hap_len, dev_out,
                                  (hipStream_t)stream, stats);
+  if (stats && !rc)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int vd_grad(const vd_batch* batch, const vd_params* params, int mode, const double* weights, double* out,
+            double* grad, vd_stats* stats) {
+  if (!batch || !params || !grad) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return grad_host_locked(batch, params, mode, weights, out, grad, stats);
+}
+
+int vd_grad_device(const vd_batch* dev_batch, const vd_params* params, int mode, int64_t npairs, int max_hap_len,
+                   int max_read_len, const double* dev_weights, double* dev_out, double* dev_grad, void* stream,
+                   vd_stats* stats) {
+  if (!dev_batch || !params || !dev_out || !dev_grad) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (int rc = engine_init()) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc = grad_device_locked(dev_batch, params, mode, npairs, max_hap_len, max_read_len, dev_weights, dev_out,
+                              dev_grad, (hipStream_t)stream, stats);
   if (stats && !rc)
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;

@@ -31,6 +31,8 @@ constexpr uint32_t NOMATCH = 0xFFFFu;   // read code of dummy rows
 constexpr int CEXP32 = 108;
 constexpr int CEXP64 = 500;   // FP64 start / tile-renormalised scale 2^500
 constexpr float FLAG32_MIN = 0x1p-100f;
+constexpr int CEXPG64 = 900;  // FP64 gradient start scale 2^900 (single tile of <= 256 rows)
+constexpr int NGRAD = 66;     // gradient layout: [0] d/dgap, [1] d/dext, [2 + q] d/derr[q]
 
 // Per-quality-bin row constants of the rescaled recurrence, for one
 // precision.  With a = 1-2g, b = 1-x, c = g/4, d = x/4 and a per-bin
@@ -45,6 +47,18 @@ struct QTab {
   T good[64], bad[64], mu[64], beta[64], cfin[64];
   int E[64];
   T gamma, gap, ext;
+};
+
+// Partial derivatives of the scaled row constants with respect to g, x and
+// e_q (rho_q held fixed: it is a power of two chosen per bin), used to turn
+// the adjoint sums into gradient terms:
+//   d good/dg = -2(1-e)/rho  d bad/dg = -2(e/3)/rho  d good/de = -a/rho  d bad/de = a/(3 rho)
+//   d mu/dx = 1/(4 rho)      d beta/dx = -g/(4 a rho) d beta/dg = (1-x)/(4 rho a^2)
+//   d cfin/dg = 1/(4 rho)    d gamma/dg = (1-x)/a^2   d gamma/dx = -g/a
+struct GradTab {
+  double dgood_dg[64], dbad_dg[64], dgood_de[64], dbad_de[64];
+  double dmu_dx[64], dbeta_dx[64], dbeta_dg[64], dcfin_dg[64];
+  double dgam_dg, dgam_dx;
 };
 
 // One read x haplotype pair addressed by global read / hap index.
@@ -101,7 +115,17 @@ namespace vd {
 size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
+void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
+                 hipStream_t st);
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
                            int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
+size_t lds_grad32(int smax);
+size_t lds_grad64(int smax);
+size_t grad_ck_stride(int smax, int fp64);
+hipError_t grad_occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
+hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
+                       const QTab<double>* tab64, const GradTab* gt, int anchored, int smax, int blocks32,
+                       int blocks64, float* ck32, double* ck64, const double* weights, double* out, double* gpart,
+                       double* grad, hipStream_t st, hipEvent_t* ev);
 }  // namespace vd

@@ -53,6 +53,37 @@ def forward(batch: RegionBatch, mode=0, precision="mixed", err=None, stats=False
     return (out, st.as_dict()) if stats else out
 
 
+def grad(batch: RegionBatch, mode=0, err=None, weights=None, stats=False):
+    """Gradient of sum_p w_p ln L_p (adjoint kernels).
+
+    Returns (ln L per pair, grad) where grad is float64[66]: d/dgap, d/dext,
+    then d/derr[q] for the 64 quality bins; with stats=True also a stats dict.
+    ``weights``: one per output slot (None = all 1)."""
+    oo = batch.out_off()
+    keep = [batch.read_bytes, batch.qual_bytes, batch.read_off, batch.hap_bytes, batch.hap_off,
+            batch.region_reads, batch.region_haps, oo]
+    keep = [a if a.size else np.zeros(1, a.dtype) for a in keep]
+    vb = _lib.VdBatch(*[_ptr(a) for a in keep], batch.nregions, batch.nreads, batch.nhaps)
+    out = np.empty(max(batch.npairs, 1), np.float64)
+    g = np.zeros(GRAD_SIZE, np.float64)
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if w.shape != (batch.npairs,):
+            raise ValueError("weights must have one entry per read x haplotype pair")
+    st = _lib.VdStats()
+    p = _lib.params(batch.gap, batch.ext, err)
+    code = _lib.lib().vd_grad(ctypes.byref(vb), ctypes.byref(p), _mode(mode),
+                              w.ctypes.data_as(_lib._dp) if w is not None else None,
+                              out.ctypes.data_as(_lib._dp), g.ctypes.data_as(_lib._dp), ctypes.byref(st))
+    _lib.check(code, "vd_grad")
+    out = out[:batch.npairs]
+    return (out, g, st.as_dict()) if stats else (out, g)
+
+
+GRAD_SIZE = 66  # d/dgap, d/dext, d/derr[0..63]
+
+
 class DeviceBatch:
     """A RegionBatch copied once into HBM; forward() runs the device pipeline only."""
 
@@ -63,17 +94,18 @@ class DeviceBatch:
         oo = batch.out_off()
         self.npairs = int(batch.npairs)
         self.max_hap = int(batch.hap_lengths().max()) if batch.nhaps else 0
+        self.max_read = int(batch.read_lengths().max()) if batch.nreads else 0
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a if a.size else np.zeros(1, a.dtype))).to(device)
         self.tensors = [t(batch.read_bytes), t(batch.qual_bytes), t(batch.read_off), t(batch.hap_bytes),
                         t(batch.hap_off), t(batch.region_reads), t(batch.region_haps), t(oo)]
         self.out = torch.empty(max(self.npairs, 1), dtype=torch.float64, device=device)
         self.vb = _lib.VdBatch(*[x.data_ptr() for x in self.tensors], batch.nregions, batch.nreads, batch.nhaps)
 
-    def forward(self, mode=0, precision="mixed", err=None, stream=None, stats=False):
+    def forward(self, mode=0, precision="mixed", err=None, stream=None, stats=False, gap=None, ext=None):
         """Enqueue the forward on `stream` (torch stream or None = torch's current stream)."""
         torch = self.torch
         s = stream if stream is not None else torch.cuda.current_stream()
-        p = _lib.params(self.batch.gap, self.batch.ext, err)
+        p = _lib.params(self.batch.gap if gap is None else gap, self.batch.ext if ext is None else ext, err)
         st = _lib.VdStats() if stats else None
         code = _lib.lib().vd_forward_device(ctypes.byref(self.vb), ctypes.byref(p), _mode(mode),
                                             _precision(precision), self.npairs, self.max_hap,
@@ -82,6 +114,28 @@ class DeviceBatch:
                                             ctypes.byref(st) if st is not None else None)
         _lib.check(code, "vd_forward_device")
         return st.as_dict() if st is not None else None
+
+    def grad(self, gap=None, ext=None, err=None, mode=0, weights=None, stream=None, stats=False):
+        """Enqueue the adjoint pass; returns the float64[66] gradient tensor on the
+        device (d/dgap, d/dext, d/derr[64]); ln L per pair lands in self.out.
+        ``weights`` is a float64 device tensor with one entry per pair (or None)."""
+        torch = self.torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        p = _lib.params(self.batch.gap if gap is None else gap, self.batch.ext if ext is None else ext, err)
+        g = torch.zeros(GRAD_SIZE, dtype=torch.float64, device=self.out.device)
+        w = None
+        if weights is not None:
+            w = weights.detach().to(device=self.out.device, dtype=torch.float64).contiguous()
+            if w.numel() != self.npairs:
+                raise ValueError("weights must have one entry per read x haplotype pair")
+        st = _lib.VdStats() if stats else None
+        code = _lib.lib().vd_grad_device(ctypes.byref(self.vb), ctypes.byref(p), _mode(mode), self.npairs,
+                                         self.max_hap, self.max_read, ctypes.c_void_p(w.data_ptr()) if w is not None
+                                         else None, ctypes.c_void_p(self.out.data_ptr()),
+                                         ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(s.cuda_stream),
+                                         ctypes.byref(st) if st is not None else None)
+        _lib.check(code, "vd_grad_device")
+        return (g, st.as_dict()) if st is not None else g
 
     def result(self) -> np.ndarray:
         return self.out[:self.npairs].cpu().numpy()
