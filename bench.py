@@ -13,6 +13,10 @@ The JSON line carries:
 * cpu_baseline -- the reference's CPU path (trie_many + pairhmm_pairs
                 fallback, compiled from the reference sources into oracle/_ref)
                 on a time-bounded sample of the same workload, on host threads
+* graph      -- the same step replayed from a captured HIP graph
+* plain      -- the one-pair-per-launch GPU forward on a sample of the workload
+* fwd_bwd    -- forward + adjoint (gradients wrt gap, ext, 64-bin error table)
+                on the 1024-region HMM-sampled workload (seed 41004, 2.05M pairs)
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--regions N]
 """
@@ -158,6 +162,64 @@ def cpu_baseline(batch, mode: int, seconds: float, threads: int):
             "gcups_1core": c1 / t1 / 1e9, "cpu_model": model}
 
 
+def plain_baseline(batch, mode: int, pairs: int):
+    """The one-pair-per-launch forward on the first `pairs` pairs of a seeded
+    random selection of regions (region order inside the selection)."""
+    from vardepth_amd import engine
+    rng = np.random.default_rng(11)
+    order = rng.permutation(batch.nregions)
+    counts = np.diff(batch.region_reads) * np.diff(batch.region_haps)
+    take, total = [], 0
+    for k in order:
+        take.append(int(k))
+        total += int(counts[k])
+        if total >= pairs:
+            break
+    sub = batch.subset(take)
+    out, ms = engine.forward_plain(sub, mode=mode, max_pairs=pairs)
+    rl, hl = sub.read_lengths(), sub.hap_lengths()
+    done = cells = 0
+    for k in range(sub.nregions):
+        for r in range(sub.region_reads[k], sub.region_reads[k + 1]):
+            for h in range(sub.region_haps[k], sub.region_haps[k + 1]):
+                if done < pairs:
+                    cells += int(rl[r]) * int(hl[h])
+                    done += 1
+    return {"gcups": round(cells / ms / 1e6, 4), "pairs": done, "cells": cells, "gpu_ms": round(ms, 3),
+            "kind": "one kernel launch per pair, one thread per read row, FP32, inputs resident in HBM"}
+
+
+def grad_bench(regions: int, steps: int, mode: int):
+    """Forward and forward+adjoint GCUPS on the HMM-sampled workload (all pairs,
+    unit weights: gradient of sum ln L wrt gap, ext and the 64-bin error table)."""
+    import torch
+    from vardepth_amd import engine, workloads
+    b = workloads.hmm_sampled_regions(seed=41004, regions=regions)
+    cells = b.cells()
+    dev = engine.DeviceBatch(b)
+    dev.forward(mode=mode)
+    dev.grad(mode=mode)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.forward(mode=mode)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) / steps * 1e3
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sts.append(dev.grad(mode=mode, stats=True)[1])
+    torch.cuda.synchronize()
+    grad_ms = (time.perf_counter() - t0) / steps * 1e3
+    k32 = float(np.mean([s_["fwd32_ms"] for s_ in sts]))
+    fwd_g = cells / fwd_ms / 1e6
+    fb_g = cells / (fwd_ms + grad_ms) / 1e6
+    return {"workload": "hmm-sampled (%d regions x 100 reads x 20 haplotypes, 150 bp, seed 41004)" % regions,
+            "pairs": b.npairs, "cells": cells, "fwd_ms": round(fwd_ms, 3), "adjoint_ms": round(grad_ms, 3),
+            "adjoint_kernel_ms": round(k32, 3), "fwd_gcups": round(fwd_g, 3), "fwd_bwd_gcups": round(fb_g, 3),
+            "ratio": round(fb_g / fwd_g, 4), "flagged": int(sts[-1]["flagged"])}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +231,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "float64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
+    ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
+    ap.add_argument("--grad-steps", type=int, default=5)
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -207,6 +273,31 @@ def main():
         ms_step, last["plan_ms"], last["fwd32_ms"], last["fwd64_ms"], last["flagged"], last["fp64_pairs"]),
         file=sys.stderr, flush=True)
 
+    graph = None
+    if not args.no_graph:
+        g = dev.graph(mode=args.mode, precision=args.precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        gms = (time.perf_counter() - t0) / args.steps * 1e3
+        graph = {"ms_per_step": round(gms, 3), "gcups": round(cells / gms / 1e6, 3), "replays": args.steps}
+        print("graph replay %.2f ms/step" % gms, file=sys.stderr, flush=True)
+        del g
+
+    plain = None
+    if args.plain_pairs > 0:
+        plain = plain_baseline(batch, args.mode, args.plain_pairs)
+        plain["speedup"] = round(gcups / plain["gcups"], 1)
+        print("plain one-pair-per-launch: %.3f GCUPS on %d pairs" % (plain["gcups"], plain["pairs"]),
+              file=sys.stderr, flush=True)
+
+    fwd_bwd = None
+    if args.grad_regions > 0:
+        fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode)
+        print("fwd+bwd: %s" % json.dumps(fwd_bwd), file=sys.stderr, flush=True)
+
     if args.copy_calib:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -236,6 +327,9 @@ def main():
                      "flops_per_cell": FLOPS_PER_CELL, "cells_per_launch": cells32,
                      "kernel_ms": round(fwd32, 3)},
         "cpu_baseline": cpu,
+        "graph": graph,
+        "plain": plain,
+        "fwd_bwd": fwd_bwd,
     }
     print(json.dumps(line), flush=True)
 

@@ -155,6 +155,17 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
                    int max_hap_len, int max_read_len, const double *dev_weights, double *dev_out,
                    double *dev_grad, void *stream, vd_stats *stats);
 
+/* ---- baseline ----------------------------------------------------------- */
+
+/* The plain GPU forward: the first max_pairs pairs of the batch (region
+ * order), one kernel launch per pair, one workgroup with one thread per read
+ * row, FP32 without rescue.  A measurement baseline for the batched engine,
+ * not a production path.  *gpu_ms: device time from the first launch to the
+ * last completion (inputs already staged).  out[i] is NaN for pairs not run.
+ * Reads of at most 256 rows. */
+int vd_forward_plain(const vd_batch *batch, const vd_params *params, int mode, int64_t max_pairs,
+                     double *out, double *gpu_ms);
+
 /* Message of the last failed call on this process. */
 const char *vd_last_error(void);
 /* Device index used by the library (VARDEPTH_CUDA_DEVICE, default 0). */

@@ -109,3 +109,44 @@ def test_output_indexing_is_exact():
     oo, oo2 = b.out_off(), b2.out_off()
     for new, k in enumerate(order):
         np.testing.assert_allclose(out2[oo2[new]:oo2[new + 1]], out[oo[k]:oo[k + 1]], rtol=0, atol=1e-10)
+
+
+def test_graph_capture_replays():
+    """The whole forward captured in a HIP graph: every replay, with eager work
+    and synchronisation in between, reproduces the eager result."""
+    import torch
+    b = workloads.uneven_regions(seed=5, regions=16)
+    dev = engine.DeviceBatch(b)
+    dev.forward(mode=0, precision="mixed")
+    torch.cuda.synchronize()
+    eager = dev.result()
+    g = dev.graph(mode=0, precision="mixed")
+    scratch = torch.zeros(1 << 20, device="cuda")
+    for rep in range(4):
+        dev.out.fill_(float("nan"))
+        scratch.add_(1.0)  # unrelated eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(dev.result(), eager, rtol=0, atol=1e-12)
+    expect, _ = oracle.regions_cpu(b, mode=0)
+    close(eager, expect, TOL_MIXED)
+
+
+def test_plain_baseline_matches():
+    """The one-pair-per-launch baseline computes the same likelihoods (FP32, no
+    rescue: only pairs inside the FP32 window are compared -- the 64x8 region in
+    semiglobal mode, and the short grid shapes in anchored mode, where 150 bp
+    reads against longer haplotypes fall below it)."""
+    pairs = fx.pairs("region")
+    R, H = (int(v) for v in G["region_shape"])
+    b = workloads.from_regions([([pairs[i * H][0] for i in range(R)], [pairs[i * H][1] for i in range(R)],
+                                 [pairs[j][2] for j in range(H)])])
+    out, ms = engine.forward_plain(b, mode=0, max_pairs=200)
+    assert np.isnan(out[200:]).all() and ms > 0
+    close(out[:200], G["region_dense_m0"][:200], TOL_MIXED)
+    grid = fx.pairs("grid")
+    short = [i for i, (s, q, h) in enumerate(grid) if 0 < len(s) <= 40]
+    gb = region_batch_from_pairs([grid[i] for i in short])
+    out1, _ = engine.forward_plain(gb, mode=1)
+    close(out1, G["grid_dense_m1"][short], TOL_MIXED)

@@ -188,3 +188,24 @@ def test_autograd_function():
     g_, x_, e_ = mod.values()
     assert mod.a.grad is not None and mod.c.grad.shape == (64,)
     assert abs(float(g_) - 0.01) < 1e-6 and abs(float(x_) - 0.2) < 1e-6
+
+
+def test_adam_recovers_parameters():
+    """200 Adam steps through the HIP forward/adjoint from a perturbed start move
+    gap, continuation and the populated error bins towards the generator's."""
+    from vardepth_amd import fit
+    true_gap, true_ext = 0.01, 0.3
+    err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
+    b = workloads.hmm_sampled_regions(seed=41004, regions=48, sample_gap=true_gap, sample_ext=true_ext)
+    dev = engine.DeviceBatch(workloads.true_pairs(b))
+    gap0, ext0, err0 = fit.perturbed_start(41004, true_gap, true_ext, err_true)
+    _, hist = fit.fit(dev, gap0, ext0, err0, steps=200, lr=0.05)
+    first, last = hist[0], hist[-1]
+    assert last["loss"] < first["loss"]
+    assert abs(last["gap"] - true_gap) < 0.3 * abs(first["gap"] - true_gap), (first["gap"], last["gap"])
+    assert abs(last["ext"] - true_ext) < 0.3 * abs(first["ext"] - true_ext), (first["ext"], last["ext"])
+    used = np.arange(10, 41)
+    le = lambda e: np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used])))
+    assert le(last["err"]) < 0.5 * le(first["err"]), (le(first["err"]), le(last["err"]))
+    print("gap %.4g -> %.4g, ext %.3g -> %.3g, log10 err %.3f -> %.3f" % (
+        first["gap"], last["gap"], first["ext"], last["ext"], le(first["err"]), le(last["err"])))

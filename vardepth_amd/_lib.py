@@ -28,7 +28,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch",
            "read_haplotype_batch_mode", "wide_capability", "anchored_pairhmm",
            "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
-           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_last_error", "vd_device")
+           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_last_error",
+           "vd_device")
 
 
 class VdParams(ctypes.Structure):
@@ -100,6 +101,9 @@ def lib() -> ctypes.CDLL:
                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.POINTER(VdStats)]
     L.vd_grad_device.restype = ctypes.c_int
+    L.vd_forward_plain.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int, ctypes.c_int64,
+                                   _dp, _dp]
+    L.vd_forward_plain.restype = ctypes.c_int
     L.vd_last_error.argtypes = []
     L.vd_last_error.restype = ctypes.c_char_p
     L.vd_device.argtypes = []

@@ -436,6 +436,68 @@ int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, cons
   return 0;
 }
 
+// ---- plain baseline: one pair per launch ---------------------------------
+
+int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int64_t max_pairs, double* out,
+                      double* gpu_ms) {
+  int64_t npairs = 0;
+  int max_hap = 0;
+  uint64_t cells = 0;
+  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  if (int rc = upload_params(*params, st)) return rc;
+  if (int rc = stage_host_batch(hb, npairs, st)) return rc;
+  static vd::QTab<float> t32;
+  static int32_t E[64];
+  build_tab(*params, t32, E);
+  const int64_t todo = std::min<int64_t>(std::max<int64_t>(max_pairs, 0), npairs);
+  std::vector<double> lnk(hb->nreads);
+  for (int64_t r = 0; r < hb->nreads; ++r) {
+    int64_t e = 0;
+    for (int64_t k = hb->read_off[r]; k < hb->read_off[r + 1]; ++k) e += E[std::min<int>(hb->qual_bytes[k], 63)];
+    lnk[r] = (double)(e - vd::CEXP32) * 0.69314718055994530942;
+  }
+  const uint8_t* dread = g.in_read.as<uint8_t>();
+  const uint8_t* dqual = g.in_qual.as<uint8_t>();
+  const uint8_t* dhap = g.in_hap.as<uint8_t>();
+  double* dout = g.in_out.as<double>();
+  std::vector<char> state(std::max<int64_t>(npairs, 1), 0);  // 1 = launched, 2 = empty pair
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventRecord(g.ev[0], st));
+  int64_t done = 0;
+  for (int64_t k = 0; k < hb->nregions && done < todo; ++k) {
+    const int64_t r0 = hb->region_reads[k], r1 = hb->region_reads[k + 1];
+    const int64_t h0 = hb->region_haps[k], h1 = hb->region_haps[k + 1];
+    for (int64_t r = r0; r < r1 && done < todo; ++r)
+      for (int64_t h = h0; h < h1 && done < todo; ++h, ++done) {
+        const int n = (int)(hb->read_off[r + 1] - hb->read_off[r]);
+        const int m = (int)(hb->hap_off[h + 1] - hb->hap_off[h]);
+        const int64_t o = hb->out_off[k] + (r - r0) * (h1 - h0) + (h - h0);
+        if (n == 0 || m == 0) {
+          state[o] = 2;
+          continue;
+        }
+        if (n + anchored > 256) return fail("vdhmm: the plain baseline handles reads of at most 256 rows");
+        HIPCHK(vd::launch_plain_pair(dread + hb->read_off[r], dqual + hb->read_off[r], n, dhap + hb->hap_off[h], m,
+                                     g.tab32.as<vd::QTab<float>>(), anchored, lnk[r], dout + o, st));
+        state[o] = 1;
+      }
+  }
+  HIPCHK(hipEventRecord(g.ev[1], st));
+  HIPCHK(hipEventSynchronize(g.ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, g.ev[0], g.ev[1]));
+  if (gpu_ms) *gpu_ms = ms;
+  std::vector<double> tmp(std::max<int64_t>(npairs, 1));
+  if (npairs) HIPCHK(hipMemcpy(tmp.data(), dout, npairs * 8, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < npairs; ++i) out[i] = state[i] == 1 ? tmp[i] : (state[i] == 2 ? -INFINITY : NAN);
+  return 0;
+}
+
 // ---- helpers for the drop-ins ---------------------------------------------
 
 struct HostBatch {
@@ -600,6 +662,13 @@ int vd_grad_device(const vd_batch* dev_batch, const vd_params* params, int mode,
   if (stats && !rc)
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+int vd_forward_plain(const vd_batch* batch, const vd_params* params, int mode, int64_t max_pairs, double* out,
+                     double* gpu_ms) {
+  if (!batch || !params || !out) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return plain_host_locked(batch, params, mode, max_pairs, out, gpu_ms);
 }
 
 const char* vd_last_error(void) { return g_err.c_str(); }

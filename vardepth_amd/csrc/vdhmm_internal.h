@@ -120,6 +120,8 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
                            int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
+hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
+                             const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
 size_t lds_grad32(int smax);
 size_t lds_grad64(int smax);
 size_t grad_ck_stride(int smax, int fp64);

@@ -84,6 +84,24 @@ def grad(batch: RegionBatch, mode=0, err=None, weights=None, stats=False):
 GRAD_SIZE = 66  # d/dgap, d/dext, d/derr[0..63]
 
 
+def forward_plain(batch: RegionBatch, mode=0, err=None, max_pairs=None):
+    """The one-pair-per-launch baseline on the first max_pairs pairs (region
+    order): returns (ln L with NaN for pairs not run, device milliseconds)."""
+    oo = batch.out_off()
+    keep = [batch.read_bytes, batch.qual_bytes, batch.read_off, batch.hap_bytes, batch.hap_off,
+            batch.region_reads, batch.region_haps, oo]
+    keep = [a if a.size else np.zeros(1, a.dtype) for a in keep]
+    vb = _lib.VdBatch(*[_ptr(a) for a in keep], batch.nregions, batch.nreads, batch.nhaps)
+    out = np.full(max(batch.npairs, 1), np.nan)
+    ms = ctypes.c_double(0.0)
+    p = _lib.params(batch.gap, batch.ext, err)
+    todo = batch.npairs if max_pairs is None else int(max_pairs)
+    code = _lib.lib().vd_forward_plain(ctypes.byref(vb), ctypes.byref(p), _mode(mode), todo,
+                                       out.ctypes.data_as(_lib._dp), ctypes.byref(ms))
+    _lib.check(code, "vd_forward_plain")
+    return out[:batch.npairs], ms.value
+
+
 class DeviceBatch:
     """A RegionBatch copied once into HBM; forward() runs the device pipeline only."""
 
@@ -114,6 +132,22 @@ class DeviceBatch:
                                             ctypes.byref(st) if st is not None else None)
         _lib.check(code, "vd_forward_device")
         return st.as_dict() if st is not None else None
+
+    def graph(self, mode=0, precision="mixed", err=None, gap=None, ext=None):
+        """Capture one forward (planning, FP32 kernel, FP64 rescue) in a HIP graph.
+
+        A first eager forward sizes the device buffers and uploads the
+        parameter tables, so the captured call only enqueues kernels (counters
+        are cleared by a kernel, not a memset).  Replaying the returned
+        torch.cuda.CUDAGraph recomputes self.out for the same parameters."""
+        torch = self.torch
+        self.forward(mode=mode, precision=precision, err=err, gap=gap, ext=ext)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        with torch.cuda.graph(g, stream=s):
+            self.forward(mode=mode, precision=precision, err=err, gap=gap, ext=ext, stream=s)
+        return g
 
     def grad(self, gap=None, ext=None, err=None, mode=0, weights=None, stream=None, stats=False):
         """Enqueue the adjoint pass; returns the float64[66] gradient tensor on the

@@ -315,56 +315,89 @@ def precision_stress(seed: int = 41003, regions: int = 1024, reads: int = 16,
     return b.batch(gap, ext)
 
 
-def _hmm_read(rng, hap: np.ndarray, n: int, qual: np.ndarray, gap: float, ext: float):
-    """Sample a read of n bases from the PairHMM: start uniformly in the haplotype,
-    M->I / M->D with probability gap, I->I / D->D with probability ext, match emission
-    1-e else a uniform other base, insertions uniform over ACGT."""
-    out = np.empty(n, dtype=np.uint8)
-    j = int(rng.integers(0, max(1, len(hap) - n + 1)))
-    state = 0  # 0=M, 1=I, 2=D
-    i = 0
+def _hmm_reads(rng, hcat: np.ndarray, hoff: np.ndarray, src: np.ndarray, n: int, qual: np.ndarray, gap: float,
+               ext: float) -> np.ndarray:
+    """Sample reads of n bases from the PairHMM, all reads at once: each starts
+    uniformly in its haplotype ``src[r]``; M->I / M->D with probability gap,
+    I->I / D->D with probability ext; match emission 1-e else a uniform other
+    base, insertions uniform over ACGT; a deletion running off the haplotype end
+    returns to M on the last base."""
+    R = len(src)
+    hl = hoff[src + 1] - hoff[src]
+    j = (rng.random(R) * np.maximum(1, hl - n + 1)).astype(np.int64)
+    state = np.zeros(R, np.int8)  # 0 = M, 1 = I, 2 = D
+    i = np.zeros(R, np.int64)
+    out = np.zeros((R, n), np.uint8)
     err = np.power(10.0, -qual.astype(np.float64) / 10.0)
-    while i < n:
-        u = rng.random()
-        if state == 0:
-            state = 1 if u < gap else (2 if u < 2 * gap else 0)
-        else:
-            state = state if u < ext else 0
-        if state == 2:
-            j += 1
-            if j >= len(hap):
-                state = 0
-            continue
-        if state == 1:
-            out[i] = BASES[rng.integers(0, 4)]
-            i += 1
-            continue
-        if j >= len(hap):
-            j = len(hap) - 1
-        base = hap[j]
-        if rng.random() < err[i]:
-            base = BASES[(np.searchsorted(BASES, base) + int(rng.integers(1, 4))) % 4]
-        out[i] = base
-        i += 1
-        j += 1
+    rows = np.arange(R)
+    for _ in range(8 * n):
+        act = i < n
+        if not act.any():
+            break
+        u = rng.random(R)
+        nxt = np.where(state == 0, np.where(u < gap, 1, np.where(u < 2 * gap, 2, 0)), np.where(u < ext, state, 0))
+        state = np.where(act, nxt, state).astype(np.int8)
+        dele = act & (state == 2)
+        j = np.where(dele, j + 1, j)
+        off_end = dele & (j >= hl)
+        state = np.where(off_end, 0, state).astype(np.int8)
+        ins = act & (state == 1)
+        mat = act & (state == 0) & ~dele
+        ii = np.minimum(i, n - 1)
+        rnd = BASES[rng.integers(0, 4, R)]
+        jj = np.clip(j, 0, hl - 1)
+        base = hcat[hoff[src] + jj]
+        sub = rng.random(R) < err[rows, ii]
+        shifted = BASES[(np.searchsorted(BASES, base) + rng.integers(1, 4, R)) % 4]
+        emit = np.where(ins, rnd, np.where(sub, shifted, base))
+        emitting = ins | mat
+        out[rows[emitting], ii[emitting]] = emit[emitting]
+        i = np.where(emitting, i + 1, i)
+        j = np.where(mat, jj + 1, j)
     return out
 
 
 def hmm_sampled_regions(seed: int = 41004, regions: int = 1024, reads: int = 100, haps: int = 20,
                         read_len: int = 150, gap: float = GAP_Q45, ext: float = EXT_Q10,
                         sample_gap: float = 0.01, sample_ext: float = 0.3) -> RegionBatch:
-    """Regions whose reads are drawn from the PairHMM itself; truth hap per read recorded."""
+    """Regions whose reads are drawn from the PairHMM itself (gap sample_gap,
+    continuation sample_ext, reference error table), true haplotype per read
+    recorded; haplotypes as in single_region (200-400 bp windows with SNVs and
+    indels); qualities uniform 10..40."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = _Builder()
+    hs_all, rcount = [], []
     for _ in range(regions):
-        length = int(rng.integers(200, 401))
-        hs = _region_haps(rng, length, haps)
-        rs, qs, truth = [], [], []
-        for _ in range(reads):
-            h = int(rng.integers(0, haps))
-            q = rng.integers(10, 41, read_len).astype(np.uint8)
-            rs.append(_hmm_read(rng, hs[h], read_len, q, sample_gap, sample_ext))
-            qs.append(q)
-            truth.append(h)
-        b.add(hs, rs, qs, truth)
+        hs_all.append(_region_haps(rng, int(rng.integers(200, 401)), haps))
+        rcount.append(reads)
+    hlen = np.array([len(h) for hs in hs_all for h in hs], np.int64)
+    hoff = np.zeros(len(hlen) + 1, np.int64)
+    np.cumsum(hlen, out=hoff[1:])
+    hcat = np.concatenate([h for hs in hs_all for h in hs])
+    nreads = regions * reads
+    local = rng.integers(0, haps, nreads)
+    src = np.repeat(np.arange(regions) * haps, reads) + local
+    qual = rng.integers(10, 41, (nreads, read_len)).astype(np.uint8)
+    seq = _hmm_reads(rng, hcat, hoff, src, read_len, qual, sample_gap, sample_ext)
+    b = _Builder()
+    for k in range(regions):
+        r0 = k * reads
+        rs = _Reads(seq[r0:r0 + reads].reshape(-1), qual[r0:r0 + reads].reshape(-1), [read_len] * reads)
+        b.add(hs_all[k], rs, None, list(local[r0:r0 + reads]))
     return b.batch(gap, ext)
+
+
+def true_pairs(batch: RegionBatch) -> RegionBatch:
+    """The batch restricted to each read's true haplotype: one region per
+    (region, haplotype) with the reads sampled from it."""
+    if batch.read_true_hap is None:
+        raise ValueError("batch has no true haplotype per read")
+    regions = []
+    for k in range(batch.nregions):
+        reads, quals, haps = batch.region(k)
+        r0, h0 = batch.region_reads[k], batch.region_haps[k]
+        truth = batch.read_true_hap[r0:batch.region_reads[k + 1]] - h0
+        for h in range(len(haps)):
+            sel = np.flatnonzero(truth == h)
+            if len(sel):
+                regions.append(([reads[i] for i in sel], [quals[i] for i in sel], [haps[h]]))
+    return from_regions(regions, batch.gap, batch.ext)
