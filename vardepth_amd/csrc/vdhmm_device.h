@@ -88,6 +88,16 @@ __device__ __forceinline__ void fill_codes(const BatchDev& b, const GroupPair& g
 }
 
 
+// Work buckets.  Region-major (the forward): bucket (c, k) = k * NCLS + c, so a
+// region's classes are processed together.  Class-major with the largest
+// class first (the gradients): bucket (c, k) = (NCLS-1-c) * nregions + k, so
+// after the scan the FP32 units of class c are the contiguous range
+// [unit_cnt[bucket_of(c, 0)], unit_cnt[bucket_of(c - 1, 0)]) that one
+// per-class kernel takes.
+__host__ __device__ __forceinline__ int64_t bucket_of(int c, int64_t k, int64_t nregions, int class_major) {
+  return class_major ? (int64_t)(NCLS - 1 - c) * nregions + k : k * NCLS + c;
+}
+
 // FP32 unit u -> the lane's group pair; returns the unit's class (rows per
 // lane), or -1 when every group of the unit is empty.  Pairs with an empty
 // haplotype get -inf here and are marked invalid.
@@ -96,8 +106,8 @@ __device__ __forceinline__ int decode_unit32(const BatchDev& b, const PlanDev& p
   const int lane = threadIdx.x & 63, g = lane >> 4;
   const int bkt = p.unit_bucket[u];
   const int64_t j = (int64_t)u - p.unit_cnt[bkt];
-  const int64_t k = bkt / NCLS;
-  const int c = bkt % NCLS;
+  const int64_t k = p.class_major ? bkt % b.nregions : bkt / NCLS;
+  const int c = p.class_major ? NCLS - 1 - (int)(bkt / b.nregions) : bkt % NCLS;
   const int64_t r0 = b.region_reads[k], h0 = b.region_haps[k];
   const int64_t H = b.region_haps[k + 1] - h0;
   const int32_t* co = p.class_off + k * (NCLS + 1);

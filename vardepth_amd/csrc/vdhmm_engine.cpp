@@ -69,12 +69,12 @@ struct Engine {
   // parameter tables
   DevBuf qE, tab32, tab64, gtab;
   // gradient scratch: checkpoints, per-block partial gradients, staged weights / gradient
-  DevBuf ck32, ck64, gpart, in_w, in_grad;
+  DevBuf ck32, ck64, gpart, bins64, in_w, in_grad;
   vd_params params_on_device;
   bool have_params = false;
   int occ32[2] = {0, 0}, occ64[2] = {0, 0};
   int occ_smax = -1;
-  int gocc32[2] = {0, 0}, gocc64[2] = {0, 0};
+  int gocc32[2][17] = {}, gocc64[2][8] = {};
   int gocc_smax = -1;
 };
 
@@ -169,10 +169,11 @@ vd::BatchDev batch_dev(const vd_batch* db) {
                       db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};
 }
 
-vd::PlanDev plan_dev(int64_t npairs) {
+vd::PlanDev plan_dev(int64_t npairs, int class_major) {
   return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
                      g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
-                     g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1)};
+                     g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
+                     class_major};
 }
 
 int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
@@ -200,7 +201,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs);
+  const vd::PlanDev p = plan_dev(npairs, 0);
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks64,
                              g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr));
@@ -346,6 +347,7 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (max_read_len + anchored > vd::GROUP * vd::MAX_R32)
     return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
   if (int rc = check_params(*params)) return rc;
+  if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
   if (int rc = upload_params(*params, st)) return rc;
   static vd::GradTab gt;
@@ -356,22 +358,35 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.gocc_smax != smax) {
     for (int a = 0; a < 2; ++a) {
-      HIPCHK(vd::grad_occupancy(a, 0, smax, &g.gocc32[a]));
-      HIPCHK(vd::grad_occupancy(a, 1, smax, &g.gocc64[a]));
+      for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::grad_occupancy(a, r, 0, smax, &g.gocc32[a][r]));
+      for (int k = 0; k < 8; ++k) HIPCHK(vd::grad_occupancy(a, 2 * (k + 1), 1, smax, &g.gocc64[a][k]));
     }
     g.gocc_smax = smax;
   }
-  const int blocks32 = std::max(1, g.cus * std::max(1, g.gocc32[anchored]));
-  const int blocks64 = std::max(1, std::min(g.cus, g.cus * std::max(1, g.gocc64[anchored])));
-  HIPCHK(g.ck32.ensure((size_t)blocks32 * vd::grad_ck_stride(smax, 0) * sizeof(float)));
-  HIPCHK(g.ck64.ensure((size_t)blocks64 * vd::grad_ck_stride(smax, 1) * sizeof(double)));
-  HIPCHK(g.gpart.ensure((size_t)(blocks32 + blocks64) * vd::NGRAD * sizeof(double)));
+  // per-class persistent grids; the checkpoint and partial-gradient areas are
+  // sized for the widest of them
+  int blocks32[17] = {0}, blocks64[8] = {0}, max_blocks = 1;
+  for (int r = 1; r <= vd::MAX_R32; ++r) {
+    blocks32[r] = std::max(1, g.cus * std::max(1, g.gocc32[anchored][r]));
+    max_blocks = std::max(max_blocks, blocks32[r]);
+  }
+  int max64 = 1;
+  for (int k = 0; k < 8; ++k) {
+    blocks64[k] = std::max(1, g.cus * std::max(1, std::min(g.gocc64[anchored][k], 2)));
+    max64 = std::max(max64, blocks64[k]);
+  }
+  max_blocks = std::max(max_blocks, max64);
+  const int64_t bin_cap = std::max<int64_t>(npairs, 1);
+  HIPCHK(g.ck32.ensure((size_t)max_blocks * vd::grad_ck_stride(smax, 0) * sizeof(float)));
+  HIPCHK(g.ck64.ensure((size_t)max64 * vd::grad_ck_stride(smax, 1) * sizeof(double)));
+  HIPCHK(g.gpart.ensure((size_t)max_blocks * vd::NGRAD * sizeof(double)));
+  HIPCHK(g.bins64.ensure((size_t)8 * bin_cap * sizeof(vd::PairRef)));
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs);
+  const vd::PlanDev p = plan_dev(npairs, 1);
   HIPCHK(vd::launch_grad(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(), g.tab64.as<vd::QTab<double>>(),
-                         g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, blocks64, g.ck32.as<float>(),
-                         g.ck64.as<double>(), dweights, dout, g.gpart.as<double>(), dgrad, st,
-                         stats ? g.ev : nullptr));
+                         g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, blocks64, max_blocks,
+                         g.ck32.as<float>(), g.ck64.as<double>(), g.bins64.as<vd::PairRef>(), bin_cap, dweights,
+                         dout, g.gpart.as<double>(), dgrad, st, stats ? g.ev : nullptr));
   if (stats) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     unsigned long long c[vd::CTR_COUNT];
@@ -410,6 +425,7 @@ int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, cons
   if (max_read + (mode == VD_MODE_ANCHORED ? 1 : 0) > vd::GROUP * vd::MAX_R32)
     return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
   if (int rc = check_params(*params)) return rc;
+  if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
   if (int rc = stage_host_batch(hb, npairs, st)) return rc;

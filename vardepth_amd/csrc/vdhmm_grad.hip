@@ -24,16 +24,28 @@
 //   Mb_i(j) = Pb_i(j) + Qb_i(j) + Db_i(j+1)
 // so lane l receives its carries from lane l+1 through DPP row_shl:1, whose
 // `old` operand injects the terminal seeds into lane 15.  The seed is
-// w / terminal, which makes every sum below a derivative of w ln L:
-//   S_good_i = sum_match Mb Q_{i-1}(j-1)   S_bad_i = sum_mismatch (same)
-//   S_mu_i   = sum Pb_i I^_i               S_beta_i = sum Qb_i I^_i
-//   T_gamma  = sum Qb D^ (gamma rows)       T_x = sum Db_i(j+1) D^_i(j)
+// w / terminal, which makes every sum below a derivative of w ln L.  Per row
+// the emission sum E_i = sum_j Mb Q_{i-1}(j-1) s (s = -1 on a match, 1/3 on a
+// mismatch, 0 past the haplotype end) gives d/de of that row's quality bin
+// (times a/rho); the other sums are per lane, the row constants being folded
+// in through closed forms of their derivatives (mu = x/(4 rho),
+// beta = (1-x) g/(4 a rho), cfin = g/(4 rho), gamma = (1-x) g/a):
+//   U = sum mu Pb I^, V = sum beta Qb I^ (real rows but the read's last)
+//                               -> d/dx: U/x - V/(1-x), d/dg: V/(g a)
+//   W = sum cfin (Pb + Qb) I^ (the read's last row) -> d/dg: W/g
+//   Tg = sum Qb D^ (gamma rows) -> d gamma/dg, d gamma/dx;  Tl (last row,
+//   anchored gamma_last = g) -> d/dg;  Tx = sum Db_i(j+1) D^_i(j) -> d/dx.
+// The emissions' d/dg, -2/a sum_rows sum_j Mb M^ (the M occupancy), comes from
+// the row-cut identity: every path emits each read base once, in M or in I,
+// so the M occupancy of the n real rows is n w - (U + V + W).
 #include "vdhmm_device.h"
 
 namespace vd {
 
-constexpr int GK = 4;        // checkpoint interval (anti-diagonal steps)
+constexpr int GK32 = 4;      // checkpoint interval (anti-diagonal steps), FP32
+constexpr int GK64 = 2;      // FP64 (twice the registers per value)
 constexpr int CKW_MAX = 3 * MAX_R32 + 2;
+constexpr int NBIN64 = 8;    // FP64 gradient row bins: R = 2, 4, ..., 16
 
 __device__ __forceinline__ float dpp_shl1(float old, float src) {
   return __builtin_bit_cast(
@@ -102,18 +114,18 @@ __device__ __forceinline__ void injections(int t, int m, T C, T& pinj, T& qinj) 
   }
 }
 
-template <typename T, int R, bool ANCH>
+template <typename T, int R, bool ANCH, int GK>
 __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, const QTab<T>* __restrict__ tab,
                                           const GradTab* __restrict__ gt, uint16_t* codes, int smax,
                                           const GroupPair gp, T* __restrict__ ck, const double* __restrict__ weights,
-                                          double* __restrict__ out, double& dg, double& dx, double* lds_de,
-                                          bool rescue) {
+                                          double* __restrict__ out, double* acc, double* lds_de, bool rescue) {
   constexpr int CKW = 3 * R + 2;
   const int lane = threadIdx.x & 63, l = lane & 15;
   const int tend = gp.m_max + 16;
   fill_codes(b, gp, codes, smax, tend + 18);
   T gd[R], bd[R], mu[R], be[R];
   uint32_t rc[R];
+  bool pass[R];  // pass-through row above the read (structural mu = beta = 1)
   const int np = gp.n + (ANCH ? 1 : 0);
   const int pad = GROUP * R - np;
   const int64_t rb = b.read_off[gp.read];
@@ -140,6 +152,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
       }
     }
     gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
+    pass[r] = rel < 0;
   }
   const T gam = tab->gamma, xx = tab->ext;
   const T gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : T(0)) : gam;
@@ -151,7 +164,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   T M[R], D[R], Q[R], qu[R], dd[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0;
-  T Plast = 0, Xprev = 0, acc = 0, rmax = 0, latch = 0;
+  T Plast = 0, Xprev = 0, term_acc = 0, rmax = 0, latch = 0;
   for (int t = 0; t <= tend; ++t) {
     if (t % GK == 0) {
       T* c = ck + (size_t)(t / GK) * CKW * 64 + lane;
@@ -174,14 +187,14 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
       rmax = fmax(rmax, Q[R - 1]);
       if (t - l - 1 == gp.m) latch = Q[R - 1];
     } else {
-      acc += Plast;
+      term_acc += Plast;
     }
   }
 
   // ---- terminal, flag and seed (lane 15 of each group holds the pair's terminal)
-  const T term = ANCH ? latch : acc;
+  const T term = ANCH ? latch : term_acc;
   bool flag = false;
-  double seed = 0.0;
+  double seed = 0.0, wpair = 0.0;
   if (l == GROUP - 1 && gp.valid) {
     const double lnk = (double)(p.read_exp[gp.read] - GradScale<T>::CEXP) * LN2;
     double v;
@@ -190,7 +203,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
       const double ll = log((double)latch);
       v = (ll > lc ? ll : lc) + lnk;
     } else {
-      v = log((double)acc) + lnk - log((double)(gp.m + 1));
+      v = log((double)term_acc) + lnk - log((double)(gp.m + 1));
     }
     if (rescue) {
       flag = !((double)term > 0.0) || !((double)term < __builtin_inf());
@@ -205,8 +218,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
       }
     }
     out[gp.out] = flag && !rescue ? __builtin_nan("") : v;
-    const double w = weights ? weights[gp.out] : 1.0;
-    seed = flag ? 0.0 : w / (double)term;
+    wpair = weights ? weights[gp.out] : 1.0;
+    seed = flag ? 0.0 : wpair / (double)term;
   }
   const int src = (lane & ~15) + 15;
   seed = __shfl(seed, src);
@@ -214,19 +227,43 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   const T tseed = (T)seed;
 
   // ---- sweep 2: backward, block by block
-  T EMp[R], Dbr[R], Am[R], Ax[R], Smu[R], Sbe[R];
+  T EMp[R], Dbr[R], Er[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) EMp[r] = Dbr[r] = Am[r] = Ax[r] = Smu[r] = Sbe[r] = 0;
-  T Tg = 0, Tlast = 0, Txd = 0, Ilast = 0;
+  for (int r = 0; r < R; ++r) EMp[r] = Dbr[r] = Er[r] = 0;
+  T U = 0, V = 0, UL = 0, VL = 0, Tg = 0, Tl = 0, Tx = 0, Ilast = 0;
   // the anchored terminal of a pair whose haplotype is the unit's longest
   // enters before the first backward step
   T Eprev = (ANCH && l == GROUP - 1 && tend - 16 == gp.m) ? tseed : T(0);
   const T Pseed = ANCH ? T(0) : tseed;
   const int nblk = tend / GK + 1;
+  // Where the kernel already runs one wave per SIMD, the previous block's
+  // checkpoint is loaded while the current block is processed (the extra
+  // registers cost no occupancy there, and nothing else would hide the load).
+  constexpr bool PREF = sizeof(T) == 4 ? R >= 9 : R >= 8;
+  T ckn[PREF ? CKW : 1];
+  if (PREF) {
+    const T* c = ck + (size_t)(nblk - 1) * CKW * 64 + lane;
+#pragma unroll
+    for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
+  }
   for (int blk = nblk - 1; blk >= 0; --blk) {
     const int t0 = blk * GK;
     const int cnt = min(GK, tend + 1 - t0);
-    {
+    if (PREF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        M[r] = ckn[r];
+        D[r] = ckn[(R + r) % (PREF ? CKW : 1)];
+        Q[r] = ckn[(2 * R + r) % (PREF ? CKW : 1)];
+      }
+      Plast = ckn[(3 * R) % (PREF ? CKW : 1)];
+      Xprev = ckn[(3 * R + 1) % (PREF ? CKW : 1)];
+      if (blk > 0) {
+        const T* c = ck + (size_t)(blk - 1) * CKW * 64 + lane;
+#pragma unroll
+        for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
+      }
+    } else {
       const T* c = ck + (size_t)blk * CKW * 64 + lane;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -257,19 +294,21 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
         const int t = t0 + s;
         const uint32_t code = cl[t];
         const bool post = code == SENT_POST;
+        const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
         const T Pbin = dpp_shl1(Pseed, Ilast);
         const T Qbin = Eprev;
         const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
         const T En = dpp_shl1(einj, EMp[0]);
-        T Qb[R], e[R], Iv[R];
+        T Qb[R], e[R], sw[R], Iv[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
         T Ih = sPin[s];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          T er = rc[r] == code ? gd[r] : bd[r];
-          e[r] = post ? T(0) : er;
-          Iv[r] = Ih;
+          const bool mt = rc[r] == code;
+          e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
+          sw[r] = mt ? T(-1) : smis;
+          Iv[r] = pass[r] ? T(0) : Ih;
           Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
         }
         T Pb = Pbin;
@@ -278,19 +317,22 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           const T gr = r == R - 1 ? gam_last : gam;
           const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
           const T Mbar = Pb + Qb[r] + Dbr[r];
-          const T mq = Mbar * sQ[s][r];
-          const bool mt = rc[r] == code;
-          Am[r] += mt ? mq : T(0);
-          Ax[r] += (mt || post) ? T(0) : mq;
-          Smu[r] = fmaT(Pb, Iv[r], Smu[r]);
-          Sbe[r] = fmaT(Qb[r], Iv[r], Sbe[r]);
-          const T dq = Qb[r] * sD[s][r];
-          if (r == R - 1) Tlast += dq; else Tg += dq;
-          Txd = fmaT(Dbr[r], sD[s][r], Txd);
-          const T Ibar = fmaT(be[r], Qb[r], mu[r] * Pb);
+          const T EM = e[r] * Mbar;
+          Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
+          const T u = mu[r] * Pb, v = be[r] * Qb[r];
+          if (r == R - 1) {
+            UL = fmaT(u, Iv[r], UL);
+            VL = fmaT(v, Iv[r], VL);
+            Tl = fmaT(Qb[r], sD[s][r], Tl);
+          } else {
+            U = fmaT(u, Iv[r], U);
+            V = fmaT(v, Iv[r], V);
+            Tg = fmaT(Qb[r], sD[s][r], Tg);
+          }
+          Tx = fmaT(Dbr[r], sD[s][r], Tx);
           Dbr[r] = Dbar;
-          EMp[r] = e[r] * Mbar;
-          Pb = Ibar;
+          EMp[r] = EM;
+          Pb = u + v;
         }
         Ilast = Pb;
         Eprev = En;
@@ -298,39 +340,34 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
     }
   }
 
-  // ---- sweep 3: gradient terms of the pair's rows
+  // ---- sweep 3: gradient terms of the pair
   if (gp.valid && !flag) {
-    double lg = 0.0, lx = 0.0;
+    const double g = (double)tab->gap, x = (double)tab->ext, a = 1.0 - 2.0 * g;
+    const bool last_lane = l == GROUP - 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int rel = l * R + r - pad;
-      const int kk = rel - (ANCH ? 1 : 0);
+      const int kk = l * R + r - pad - (ANCH ? 1 : 0);
       if (kk >= 0) {
         int q = b.qual_bytes[rb + kk];
         q = q < 63 ? q : 63;
-        const double am = Am[r], ax = Ax[r], smu = Smu[r], sbe = Sbe[r];
-        atomicAdd(&lds_de[q], am * gt->dgood_de[q] + ax * gt->dbad_de[q]);
-        lg += am * gt->dgood_dg[q] + ax * gt->dbad_dg[q];
-        if (kk == gp.n - 1) {
-          lg += (smu + sbe) * gt->dcfin_dg[q];
-        } else {
-          lx += smu * gt->dmu_dx[q] + sbe * gt->dbeta_dx[q];
-          lg += sbe * gt->dbeta_dg[q];
-        }
+        atomicAdd(&lds_de[q], (double)Er[r] * gt->dgood_de[q] * -1.0);  // dgood_de = -a/rho
       }
     }
-    // register R-1 of lane 15 is the read's last row (gamma_last); every other
-    // row uses gamma
-    const double tg = (double)Tg + (l == GROUP - 1 ? 0.0 : (double)Tlast);
-    const double tl = l == GROUP - 1 ? (double)Tlast : 0.0;
-    lg += tg * gt->dgam_dg + (ANCH ? tl : 0.0);
-    lx += tg * gt->dgam_dx + (double)Txd;
-    dg += lg;
-    dx += lx;
+    const double u = (double)U + (last_lane ? 0.0 : (double)UL);
+    const double v = (double)V + (last_lane ? 0.0 : (double)VL);
+    const double w = last_lane ? (double)UL + (double)VL : 0.0;
+    const double tg = (double)Tg + (last_lane ? 0.0 : (double)Tl);
+    const double tl = last_lane ? (double)Tl : 0.0;
+    const double mocc = last_lane ? (double)gp.n * wpair : 0.0;  // n w, once per pair
+    acc[0] += -2.0 / a * (mocc - (u + v + w)) + v / (g * a) + w / g + tg * gt->dgam_dg + (ANCH ? tl : 0.0);
+    acc[1] += u / x - v / (1.0 - x) + tg * gt->dgam_dx + (double)Tx;
   }
   __syncthreads();  // codes reused by the next unit
 }
 
+// Adds the block's gradient (lane sums of d/dg, d/dx and the LDS per-bin
+// d/de) into its partial row; the per-class kernels run one after another on
+// one stream, so each adds to the rows the previous ones left.
 __device__ __forceinline__ void grad_flush(double dg, double dx, const double* lds_de, double* __restrict__ gpart) {
   for (int o = 32; o > 0; o >>= 1) {
     dg += __shfl_xor(dg, o);
@@ -339,79 +376,91 @@ __device__ __forceinline__ void grad_flush(double dg, double dx, const double* l
   __syncthreads();
   double* gp = gpart + (size_t)blockIdx.x * NGRAD;
   if (threadIdx.x == 0) {
-    gp[0] = dg;
-    gp[1] = dx;
+    gp[0] += dg;
+    gp[1] += dx;
   }
-  for (int q = threadIdx.x; q < 64; q += blockDim.x) gp[2 + q] = lds_de[q];
+  for (int q = threadIdx.x; q < 64; q += blockDim.x) gp[2 + q] += lds_de[q];
 }
 
-template <bool ANCH>
+template <typename T>
+__device__ __forceinline__ void grad_lds_init(const void* gtab, size_t tab_bytes, unsigned char* lds) {
+  double* lds_de = reinterpret_cast<double*>(lds);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(lds + 64 * sizeof(double));
+  for (int i = threadIdx.x; i < (int)(tab_bytes / 4); i += blockDim.x) dst[i] = src[i];
+  for (int q = threadIdx.x; q < 64; q += blockDim.x) lds_de[q] = 0.0;
+  __syncthreads();
+}
+
+// FP32 adjoint of the units of row class R: the contiguous unit range of the
+// class in the class-major plan, taken from a per-class counter.
+template <bool ANCH, int R>
 __global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                                const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
                                                size_t ck_stride, const double* __restrict__ weights,
                                                double* __restrict__ out, double* __restrict__ gpart) {
   extern __shared__ __align__(16) unsigned char lds[];
+  grad_lds_init<float>(gtab, sizeof(QTab<float>), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
-  QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds + 64 * sizeof(double));
+  const QTab<float>* tab = reinterpret_cast<const QTab<float>*>(lds + 64 * sizeof(double));
   uint16_t* codes =
       reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
-    for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
-    for (int q = threadIdx.x; q < 64; q += blockDim.x) lds_de[q] = 0.0;
-  }
-  __syncthreads();
   float* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63;
-  double dg = 0.0, dx = 0.0;
-  const unsigned long long total = p.ctr[CTR_NUNITS];
+  double acc[2] = {0.0, 0.0};
+  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
+  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   for (;;) {
     unsigned long long u = 0;
-    if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT32], 1ull);
+    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
     u = __shfl(u, 0);
-    if (u >= total) break;
+    if (u >= hi) break;
     GroupPair gp;
-    const int c = decode_unit32(b, p, u, gp, out);
-    if (c < 0) continue;
-    switch (c) {
-#define VD_CASE(RR)                                                                                        \
-  case RR:                                                                                                 \
-    unit_grad<float, RR, ANCH>(b, p, tab, gt, codes, smax, gp, ck, weights, out, dg, dx, lds_de, false); \
-    break;
-      VD_CASE(1) VD_CASE(2) VD_CASE(3) VD_CASE(4) VD_CASE(5) VD_CASE(6) VD_CASE(7) VD_CASE(8)
-      VD_CASE(9) VD_CASE(10) VD_CASE(11) VD_CASE(12) VD_CASE(13) VD_CASE(14) VD_CASE(15) VD_CASE(16)
-#undef VD_CASE
-      default: break;
-    }
+    if (decode_unit32(b, p, u, gp, out) != R) continue;
+    unit_grad<float, R, ANCH, GK32>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false);
   }
-  grad_flush(dg, dx, lds_de, gpart);
+  grad_flush(acc[0], acc[1], lds_de, gpart);
 }
 
-// FP64 gradients of the pairs the FP32 pass flagged (reads of at most 256
-// rows: the host rejects longer ones for gradients), 4 listed pairs per unit.
+// Bins the FP64 gradient list (the pairs the FP32 pass flagged) by rows per
+// lane, rounded up to an even count, so that each FP64 kernel runs with the
+// register footprint of its bin.
 template <bool ANCH>
-__global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
-                                               const GradTab* __restrict__ gt, int smax, double* __restrict__ ckall,
-                                               size_t ck_stride, const double* __restrict__ weights,
-                                               double* __restrict__ out, double* __restrict__ gpart) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  double* lds_de = reinterpret_cast<double*>(lds);
-  QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds + 64 * sizeof(double));
-  uint16_t* codes =
-      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((sizeof(QTab<double>) + 15) & ~size_t(15)));
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
-    for (int i = threadIdx.x; i < (int)(sizeof(QTab<double>) / 4); i += blockDim.x) dst[i] = src[i];
-    for (int q = threadIdx.x; q < 64; q += blockDim.x) lds_de[q] = 0.0;
-  }
-  __syncthreads();
-  double* ck = ckall + (size_t)blockIdx.x * ck_stride;
-  const int lane = threadIdx.x & 63, g = lane >> 4;
-  double dg = 0.0, dx = 0.0;
+__global__ void k_bin64(BatchDev b, PlanDev p, PairRef* __restrict__ bins, int64_t bin_cap) {
   unsigned long long count = p.ctr[CTR_F64];
   if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    const PairRef pr = p.f64_list[i];
+    const int np = (int)(b.read_off[pr.read + 1] - b.read_off[pr.read]) + (ANCH ? 1 : 0);
+    const int rows = (np + GROUP - 1) / GROUP;
+    if (rows > MAX_R32) continue;  // the host rejects such reads before launching
+    const int bin = (rows + 1) / 2 - 1;
+    const unsigned long long k = atomicAdd(&p.ctr[CTR_BIN64 + bin], 1ull);
+    if ((int64_t)k < bin_cap) bins[(size_t)bin * bin_cap + k] = pr;
+  }
+}
+
+// FP64 adjoint of one row bin of the flagged pairs, 4 listed pairs per unit.
+template <bool ANCH, int R>
+__global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
+                                               const GradTab* __restrict__ gt, int smax, double* __restrict__ ckall,
+                                               size_t ck_stride, const PairRef* __restrict__ bins, int64_t bin_cap,
+                                               const double* __restrict__ weights, double* __restrict__ out,
+                                               double* __restrict__ gpart) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  grad_lds_init<double>(gtab, sizeof(QTab<double>), lds);
+  double* lds_de = reinterpret_cast<double*>(lds);
+  const QTab<double>* tab = reinterpret_cast<const QTab<double>*>(lds + 64 * sizeof(double));
+  uint16_t* codes =
+      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((sizeof(QTab<double>) + 15) & ~size_t(15)));
+  double* ck = ckall + (size_t)blockIdx.x * ck_stride;
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  constexpr int BIN = R / 2 - 1;
+  double acc[2] = {0.0, 0.0};
+  unsigned long long count = p.ctr[CTR_BIN64 + BIN];
+  if ((int64_t)count > bin_cap) count = (unsigned long long)bin_cap;
+  const PairRef* list = bins + (size_t)BIN * bin_cap;
   const unsigned long long units = (count + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
   for (;;) {
     unsigned long long u = 0;
@@ -421,13 +470,13 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
     const unsigned long long idx = u * PAIRS_PER_UNIT + g;
     GroupPair gp;
     gp.valid = idx < count;
-    const PairRef pr = p.f64_list[gp.valid ? idx : u * PAIRS_PER_UNIT];
+    const PairRef pr = list[gp.valid ? idx : u * PAIRS_PER_UNIT];
     gp.read = pr.read;
     gp.hap = pr.hap;
     gp.out = pr.out;
     gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
     gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
-    if (gp.valid && (gp.m == 0 || gp.n + (ANCH ? 1 : 0) > GROUP * MAX_R32)) gp.valid = 0;
+    if (gp.valid && (gp.m == 0 || gp.n + (ANCH ? 1 : 0) > GROUP * R)) gp.valid = 0;
     int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
     mx = max(mx, __shfl_xor(mx, 16));
     mx = max(mx, __shfl_xor(mx, 32));
@@ -437,9 +486,18 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
     gp.m_max = mx;
     gp.m_min = mn;
     if (!gp.valid) gp.m = mx;
-    unit_grad<double, MAX_R32, ANCH>(b, p, tab, gt, codes, smax, gp, ck, weights, out, dg, dx, lds_de, true);
+    unit_grad<double, R, ANCH, GK64>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, true);
   }
-  grad_flush(dg, dx, lds_de, gpart);
+  grad_flush(acc[0], acc[1], lds_de, gpart);
+}
+
+__global__ void k_zero_f64(double* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
+__global__ void k_reset_unit64(unsigned long long* ctr) {
+  if (threadIdx.x == 0) ctr[CTR_UNIT64] = 0ull;
 }
 
 // Sum of the per-block partial gradients in block order (deterministic for a
@@ -462,44 +520,99 @@ size_t lds_grad64(int smax) {
          (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
+// checkpoint area of one block, in values of the kernel's precision
 size_t grad_ck_stride(int smax, int fp64) {
-  const size_t blocks = (size_t)(smax / GK + 2);
-  (void)fp64;
-  return blocks * CKW_MAX * 64;
+  const int gk = fp64 ? GK64 : GK32;
+  return (size_t)(smax / gk + 2) * CKW_MAX * 64;
 }
 
-hipError_t grad_occupancy(int anchored, int fp64, int smax, int* blocks_per_cu) {
-  if (fp64)
-    return anchored
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad64<true>, 64, lds_grad64(smax))
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad64<false>, 64, lds_grad64(smax));
-  return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad32<true>, 64, lds_grad32(smax))
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad32<false>, 64, lds_grad32(smax));
+template <bool ANCH>
+static hipError_t occ_grad_r(int r, int fp64, int smax, int* n) {
+  switch (fp64 ? r / 2 + 16 : r) {
+#define VD_OCC32(RR) \
+  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax));
+    VD_OCC32(1) VD_OCC32(2) VD_OCC32(3) VD_OCC32(4) VD_OCC32(5) VD_OCC32(6) VD_OCC32(7) VD_OCC32(8)
+    VD_OCC32(9) VD_OCC32(10) VD_OCC32(11) VD_OCC32(12) VD_OCC32(13) VD_OCC32(14) VD_OCC32(15) VD_OCC32(16)
+#undef VD_OCC32
+#define VD_OCC64(RR) \
+  case 16 + RR / 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad64<ANCH, RR>, 64, lds_grad64(smax));
+    VD_OCC64(2) VD_OCC64(4) VD_OCC64(6) VD_OCC64(8) VD_OCC64(10) VD_OCC64(12) VD_OCC64(14) VD_OCC64(16)
+#undef VD_OCC64
+    default: *n = 1; return hipSuccess;
+  }
+}
+
+// blocks per CU of the FP32 kernel of class r (1..16), or of the FP64 kernel
+// of the bin with r rows per lane (r = 2, 4, ..., 16) when fp64
+hipError_t grad_occupancy(int anchored, int r, int fp64, int smax, int* blocks_per_cu) {
+  return anchored ? occ_grad_r<true>(r, fp64, smax, blocks_per_cu) : occ_grad_r<false>(r, fp64, smax, blocks_per_cu);
+}
+
+template <bool ANCH>
+static void launch_grad32_r(int r, dim3 grid, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
+                            const QTab<float>* tab32, const GradTab* gt, int smax, float* ck, size_t s32,
+                            const double* w, double* out, double* gpart) {
+  switch (r) {
+#define VD_L32(RR)                                                                                               \
+  case RR:                                                                                                       \
+    hipLaunchKernelGGL((k_grad32<ANCH, RR>), grid, dim3(64), lds, st, b, p, tab32, gt, smax, ck, s32, w, out, gpart); \
+    break;
+    VD_L32(1) VD_L32(2) VD_L32(3) VD_L32(4) VD_L32(5) VD_L32(6) VD_L32(7) VD_L32(8)
+    VD_L32(9) VD_L32(10) VD_L32(11) VD_L32(12) VD_L32(13) VD_L32(14) VD_L32(15) VD_L32(16)
+#undef VD_L32
+    default: break;
+  }
+}
+
+template <bool ANCH>
+static void launch_grad64_bin(int bin, dim3 grid, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
+                              const QTab<double>* tab64, const GradTab* gt, int smax, double* ck, size_t s64,
+                              const PairRef* bins, int64_t bin_cap, const double* w, double* out, double* gpart) {
+  switch (bin) {
+#define VD_L64(BB)                                                                                                  \
+  case BB:                                                                                                          \
+    hipLaunchKernelGGL((k_grad64<ANCH, 2 * (BB + 1)>), grid, dim3(64), lds, st, b, p, tab64, gt, smax, ck, s64, bins, \
+                       bin_cap, w, out, gpart);                                                                     \
+    break;
+    VD_L64(0) VD_L64(1) VD_L64(2) VD_L64(3) VD_L64(4) VD_L64(5) VD_L64(6) VD_L64(7)
+#undef VD_L64
+    default: break;
+  }
 }
 
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                       const QTab<double>* tab64, const GradTab* gt, int anchored, int smax, int blocks32,
-                       int blocks64, float* ck32, double* ck64, const double* weights, double* out, double* gpart,
+                       const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
+                       const int* blocks32, const int* blocks64, int max_blocks, float* ck32, double* ck64,
+                       PairRef* bins, int64_t bin_cap, const double* weights, double* out, double* gpart,
                        double* grad, hipStream_t st, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, 0, out, st);
+  hipLaunchKernelGGL(k_zero_f64, dim3(64), dim3(256), 0, st, gpart, (int64_t)max_blocks * NGRAD);
   if (ev) (void)hipEventRecord(ev[1], st);
   const size_t s32 = grad_ck_stride(smax, 0), s64 = grad_ck_stride(smax, 1);
-  if (anchored)
-    hipLaunchKernelGGL(k_grad32<true>, dim3(blocks32), dim3(64), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
-                       s32, weights, out, gpart);
-  else
-    hipLaunchKernelGGL(k_grad32<false>, dim3(blocks32), dim3(64), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
-                       s32, weights, out, gpart);
+  for (int r = MAX_R32; r >= 1; --r) {
+    if (anchored)
+      launch_grad32_r<true>(r, dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32, s32, weights,
+                            out, gpart);
+    else
+      launch_grad32_r<false>(r, dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32, s32, weights,
+                             out, gpart);
+  }
   if (ev) (void)hipEventRecord(ev[2], st);
-  double* gpart64 = gpart + (size_t)blocks32 * NGRAD;
   if (anchored)
-    hipLaunchKernelGGL(k_grad64<true>, dim3(blocks64), dim3(64), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64,
-                       s64, weights, out, gpart64);
+    hipLaunchKernelGGL(k_bin64<true>, dim3(256), dim3(256), 0, st, b, p, bins, bin_cap);
   else
-    hipLaunchKernelGGL(k_grad64<false>, dim3(blocks64), dim3(64), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64,
-                       s64, weights, out, gpart64);
-  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(128), 0, st, (const double*)gpart, blocks32 + blocks64, grad);
+    hipLaunchKernelGGL(k_bin64<false>, dim3(256), dim3(256), 0, st, b, p, bins, bin_cap);
+  for (int bin = 0; bin < NBIN64; ++bin) {
+    hipLaunchKernelGGL(k_reset_unit64, dim3(1), dim3(64), 0, st, p.ctr);
+    if (anchored)
+      launch_grad64_bin<true>(bin, dim3(blocks64[bin]), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64, s64, bins,
+                              bin_cap, weights, out, gpart);
+    else
+      launch_grad64_bin<false>(bin, dim3(blocks64[bin]), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64, s64,
+                               bins, bin_cap, weights, out, gpart);
+  }
+  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(128), 0, st, (const double*)gpart, max_blocks, grad);
   if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }

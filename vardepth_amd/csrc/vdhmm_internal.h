@@ -90,6 +90,7 @@ struct PlanDev {
   PairRef* f64_list;        // [capacity] FP64 work list
   unsigned long long* ctr;  // counters, see CTR_*
   int64_t f64_cap;
+  int class_major;          // bucket order: 0 region-major (forward), 1 class-major (gradients)
 };
 
 enum {
@@ -100,7 +101,9 @@ enum {
   CTR_OVERFLOW = 4, // FP64 list overflow (pairs dropped: reported as error)
   CTR_NUNITS = 5,   // total FP32 units (written by the scan)
   CTR_F64BAD = 6,   // FP64 results that were not finite and positive
-  CTR_COUNT = 8
+  CTR_CLS = 8,      // [8 + c]: next unit of class c (per-class gradient kernels)
+  CTR_BIN64 = 32,   // [32 + b]: FP64 gradient sub-list lengths (8 row bins)
+  CTR_COUNT = 40
 };
 
 struct LaunchCfg {
@@ -116,7 +119,7 @@ size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
-                 hipStream_t st);
+                 hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
                            int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
@@ -125,9 +128,10 @@ hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, co
 size_t lds_grad32(int smax);
 size_t lds_grad64(int smax);
 size_t grad_ck_stride(int smax, int fp64);
-hipError_t grad_occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
+hipError_t grad_occupancy(int anchored, int r, int fp64, int smax, int* blocks_per_cu);
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                       const QTab<double>* tab64, const GradTab* gt, int anchored, int smax, int blocks32,
-                       int blocks64, float* ck32, double* ck64, const double* weights, double* out, double* gpart,
+                       const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
+                       const int* blocks32, const int* blocks64, int max_blocks, float* ck32, double* ck64,
+                       PairRef* bins, int64_t bin_cap, const double* weights, double* out, double* gpart,
                        double* grad, hipStream_t st, hipEvent_t* ev);
 }  // namespace vd

@@ -111,7 +111,7 @@ __global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
     if (threadIdx.x < NCLS) {
       int c = threadIdx.x;
       int64_t units = (c >= 1 && c <= MAX_R32) ? ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT : 0;
-      p.unit_cnt[k * NCLS + c] = units;
+      p.unit_cnt[bucket_of(c, k, b.nregions, p.class_major)] = units;
     }
     __syncthreads();
   }
