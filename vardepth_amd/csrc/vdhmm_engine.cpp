@@ -72,7 +72,7 @@ struct Engine {
   DevBuf ck32, ck64, gpart, bins64, in_w, in_grad;
   vd_params params_on_device;
   bool have_params = false;
-  int occ32[2] = {0, 0}, occ64[2] = {0, 0};
+  int occ32[2][17] = {}, occ64[2] = {0, 0};
   int occ_smax = -1;
   int gocc32[2][17] = {}, gocc64[2][8] = {};
   int gocc_smax = -1;
@@ -192,16 +192,17 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.occ_smax != smax) {
     for (int a = 0; a < 2; ++a) {
-      HIPCHK(vd::occupancy(a, 0, smax, &g.occ32[a]));
-      HIPCHK(vd::occupancy(a, 1, smax, &g.occ64[a]));
+      for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::occupancy(a, r, smax, &g.occ32[a][r]));
+      HIPCHK(vd::occupancy(a, 0, smax, &g.occ64[a]));
     }
     g.occ_smax = smax;
   }
-  const int blocks32 = std::max(1, g.cus * std::max(1, g.occ32[anchored]));
+  int blocks32[17] = {0};
+  for (int r = 1; r <= vd::MAX_R32; ++r) blocks32[r] = std::max(1, g.cus * std::max(1, g.occ32[anchored][r]));
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs, 0);
+  const vd::PlanDev p = plan_dev(npairs, 1);
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks64,
                              g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr));

@@ -117,11 +117,11 @@ struct LaunchCfg {
 namespace vd {
 size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
-hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu);
+hipError_t occupancy(int anchored, int r, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                           const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
+                           const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);

@@ -281,7 +281,10 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
   __syncthreads();  // codes reused by the next unit
 }
 
-template <bool ANCH>
+// FP32 forward of the units of row class R (one kernel per class, so that each
+// runs with its own register footprint): the class's contiguous unit range in
+// the class-major plan, taken from a per-class counter.
+template <bool ANCH, int R>
 __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                               int smax, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -293,27 +296,19 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
     for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, g = lane >> 4;
-  const unsigned long long total = p.ctr[CTR_NUNITS];
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
+  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   for (;;) {
     unsigned long long u = 0;
-    if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT32], 1ull);
+    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
     u = __shfl(u, 0);
-    if (u >= total) break;
+    if (u >= hi) break;
     GroupPair gp;
-    const int c = decode_unit32(b, p, u, gp, out);
-    if (c < 0) continue;
-    switch (c) {
-#define VD_CASE(RR) \
-  case RR: unit32<RR, ANCH>(b, p, tab, codes, smax, gp, out); break;
-      VD_CASE(1) VD_CASE(2) VD_CASE(3) VD_CASE(4) VD_CASE(5) VD_CASE(6) VD_CASE(7) VD_CASE(8)
-      VD_CASE(9) VD_CASE(10) VD_CASE(11) VD_CASE(12) VD_CASE(13) VD_CASE(14) VD_CASE(15) VD_CASE(16)
-#undef VD_CASE
-      default: break;
-    }
+    if (decode_unit32(b, p, u, gp, out) != R) continue;
+    unit32<R, ANCH>(b, p, tab, codes, smax, gp, out);
   }
 }
-
 
 // ---------------------------------------------------------------------------
 // FP64: the FP32 layout (4 pairs per wave, 16-lane groups, DPP row_shr:1) with
@@ -514,8 +509,7 @@ __global__ __launch_bounds__(64) void k_fwd64(BatchDev b, PlanDev p, const QTab<
 template __global__ void k_fwd64<false>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
 template __global__ void k_fwd64<true>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
 
-template __global__ void k_fwd32<false>(BatchDev, PlanDev, const QTab<float>*, int, double*);
-template __global__ void k_fwd32<true>(BatchDev, PlanDev, const QTab<float>*, int, double*);
+
 
 }  // namespace vd
 
@@ -535,12 +529,39 @@ size_t lds_fwd64(int smax) {
   return ((sizeof(QTab<double>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
-hipError_t occupancy(int anchored, int fp64, int smax, int* blocks_per_cu) {
-  if (fp64)
+template <bool ANCH>
+static hipError_t occ_fwd32(int r, int smax, int* n) {
+  switch (r) {
+#define VD_OCC(RR) \
+  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32<ANCH, RR>, 64, lds_fwd32(smax));
+    VD_OCC(1) VD_OCC(2) VD_OCC(3) VD_OCC(4) VD_OCC(5) VD_OCC(6) VD_OCC(7) VD_OCC(8)
+    VD_OCC(9) VD_OCC(10) VD_OCC(11) VD_OCC(12) VD_OCC(13) VD_OCC(14) VD_OCC(15) VD_OCC(16)
+#undef VD_OCC
+    default: *n = 1; return hipSuccess;
+  }
+}
+
+// blocks per CU of the FP32 kernel of class r (1..16), or of the FP64 kernel (r = 0)
+hipError_t occupancy(int anchored, int r, int smax, int* blocks_per_cu) {
+  if (r == 0)
     return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<true>, 64, lds_fwd64(smax))
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<false>, 64, lds_fwd64(smax));
-  return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd32<true>, 64, lds_fwd32(smax))
-                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd32<false>, 64, lds_fwd32(smax));
+  return anchored ? occ_fwd32<true>(r, smax, blocks_per_cu) : occ_fwd32<false>(r, smax, blocks_per_cu);
+}
+
+template <bool ANCH>
+static void launch_fwd32_r(int r, int blocks, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
+                           const QTab<float>* tab32, int smax, double* out) {
+  switch (r) {
+#define VD_L(RR)                                                                                      \
+  case RR:                                                                                            \
+    hipLaunchKernelGGL((k_fwd32<ANCH, RR>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
+    break;
+    VD_L(1) VD_L(2) VD_L(3) VD_L(4) VD_L(5) VD_L(6) VD_L(7) VD_L(8)
+    VD_L(9) VD_L(10) VD_L(11) VD_L(12) VD_L(13) VD_L(14) VD_L(15) VD_L(16)
+#undef VD_L
+    default: break;
+  }
 }
 
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
@@ -562,16 +583,18 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 }
 
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                           const QTab<double>* tab64, int anchored, int force64, int smax, int blocks32,
+                           const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, force64, out, st);
   if (ev) (void)hipEventRecord(ev[1], st);
-  if (!force64 && blocks32 > 0) {
-    if (anchored)
-      hipLaunchKernelGGL(k_fwd32<true>, dim3(blocks32), dim3(64), lds_fwd32(smax), st, b, p, tab32, smax, out);
-    else
-      hipLaunchKernelGGL(k_fwd32<false>, dim3(blocks32), dim3(64), lds_fwd32(smax), st, b, p, tab32, smax, out);
+  if (!force64) {
+    for (int r = MAX_R32; r >= 1; --r) {  // largest class first
+      if (anchored)
+        launch_fwd32_r<true>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+      else
+        launch_fwd32_r<false>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+    }
   }
   if (ev) (void)hipEventRecord(ev[2], st);
   if (anchored)
