@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-forward", action="store_true",
+                    help="skip the forward workload (profiling the fwd+bwd workload alone); prints no JSON line")
     ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
@@ -245,6 +247,10 @@ def main():
 
     import torch
     from vardepth_amd import engine
+
+    if args.no_forward:
+        print(json.dumps({"fwd_bwd": grad_bench(args.grad_regions, args.grad_steps, args.mode)}), flush=True)
+        return
 
     t0 = time.perf_counter()
     batch, wname = workload(args.workload, args.regions)

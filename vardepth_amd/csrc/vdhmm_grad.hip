@@ -6,13 +6,17 @@
 // unit (4 pairs of one region and row class, 16 lanes per pair, R rows per
 // lane), processed by one wave in three sweeps:
 //
-// 1. forward over all anti-diagonal steps, writing the lane state (M^, D^, Q
-//    of its R rows and the two DPP carries) to a per-wave checkpoint area
-//    every GK steps, and computing each pair's terminal;
-// 2. backward over the steps in reverse, block by block: a block of GK steps
-//    is recomputed from its checkpoint with the per-step values the adjoint
-//    needs (each row's diagonal source Q_{i-1}(j-1), its D^, the lane's
-//    injected P) kept in registers, then swept backwards;
+// 1. forward over all anti-diagonal steps, writing the lane state to a
+//    per-wave checkpoint area every GK * NSUB steps: for each of its R rows
+//    x D^ + M^ (the next step's D^, the only use of the pair) and Q, plus the
+//    two DPP carries; and computing each pair's terminal;
+// 2. backward over the steps in reverse, checkpoint block by block, each in
+//    NSUB register blocks of GK steps taken last to first: a register block
+//    is recomputed from the checkpoint (the steps before it without storing)
+//    with the per-step values the adjoint needs (each row's diagonal source
+//    Q_{i-1}(j-1), its D^, the lane's injected P) kept in registers, then swept
+//    backwards.  The checkpoint traffic is (2R + 2) values per lane every
+//    GK * NSUB steps, at the price of recomputing (NSUB + 1) / 2 times;
 // 3. per row, the accumulated adjoint sums are turned into gradient terms
 //    through tables of partial derivatives of the scaled row constants.
 //
@@ -42,9 +46,10 @@
 
 namespace vd {
 
-constexpr int GK32 = 4;      // checkpoint interval (anti-diagonal steps), FP32
+constexpr int GK32 = 2;      // register block (anti-diagonal steps), FP32
 constexpr int GK64 = 2;      // FP64 (twice the registers per value)
-constexpr int CKW_MAX = 3 * MAX_R32 + 2;
+constexpr int NSUB = 2;      // register blocks per checkpoint
+constexpr int CKW_MAX = 2 * MAX_R32 + 2;
 constexpr int NBIN64 = 8;    // FP64 gradient row bins: R = 2, 4, ..., 16
 
 __device__ __forceinline__ float dpp_shl1(float old, float src) {
@@ -119,7 +124,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
                                           const GradTab* __restrict__ gt, uint16_t* codes, int smax,
                                           const GroupPair gp, T* __restrict__ ck, const double* __restrict__ weights,
                                           double* __restrict__ out, double* acc, double* lds_de, bool rescue) {
-  constexpr int CKW = 3 * R + 2;
+  constexpr int CKW = 2 * R + 2;
   const int lane = threadIdx.x & 63, l = lane & 15;
   const int tend = gp.m_max + 16;
   fill_codes(b, gp, codes, smax, tend + 18);
@@ -165,17 +170,17 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
 #pragma unroll
   for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0;
   T Plast = 0, Xprev = 0, term_acc = 0, rmax = 0, latch = 0;
+  constexpr int CK_STEPS = GK * NSUB;
   for (int t = 0; t <= tend; ++t) {
-    if (t % GK == 0) {
-      T* c = ck + (size_t)(t / GK) * CKW * 64 + lane;
+    if (t % CK_STEPS == 0) {
+      T* c = ck + (size_t)(t / CK_STEPS) * CKW * 64 + lane;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        c[(size_t)r * 64] = M[r];
-        c[(size_t)(R + r) * 64] = D[r];
-        c[(size_t)(2 * R + r) * 64] = Q[r];
+        c[(size_t)r * 64] = fmaT(xx, D[r], M[r]);
+        c[(size_t)(R + r) * 64] = Q[r];
       }
-      c[(size_t)(3 * R) * 64] = Plast;
-      c[(size_t)(3 * R + 1) * 64] = Xprev;
+      c[(size_t)(2 * R) * 64] = Plast;
+      c[(size_t)(2 * R + 1) * 64] = Xprev;
     }
     T pinj, qinj;
     injections<T, ANCH>(t, gp.m, C, pinj, qinj);
@@ -235,29 +240,21 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   // enters before the first backward step
   T Eprev = (ANCH && l == GROUP - 1 && tend - 16 == gp.m) ? tseed : T(0);
   const T Pseed = ANCH ? T(0) : tseed;
-  const int nblk = tend / GK + 1;
+  const int nblk = tend / CK_STEPS + 1;
   // Where the kernel already runs one wave per SIMD, the previous block's
   // checkpoint is loaded while the current block is processed (the extra
   // registers cost no occupancy there, and nothing else would hide the load).
-  constexpr bool PREF = sizeof(T) == 4 ? R >= 9 : R >= 8;
-  T ckn[PREF ? CKW : 1];
+  constexpr bool PREF = sizeof(T) == 4 ? false : R >= 8;
+  T cur[CKW], ckn[PREF ? CKW : 1];
   if (PREF) {
     const T* c = ck + (size_t)(nblk - 1) * CKW * 64 + lane;
 #pragma unroll
     for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
   }
   for (int blk = nblk - 1; blk >= 0; --blk) {
-    const int t0 = blk * GK;
-    const int cnt = min(GK, tend + 1 - t0);
     if (PREF) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        M[r] = ckn[r];
-        D[r] = ckn[(R + r) % (PREF ? CKW : 1)];
-        Q[r] = ckn[(2 * R + r) % (PREF ? CKW : 1)];
-      }
-      Plast = ckn[(3 * R) % (PREF ? CKW : 1)];
-      Xprev = ckn[(3 * R + 1) % (PREF ? CKW : 1)];
+      for (int w = 0; w < CKW; ++w) cur[w] = ckn[w % (PREF ? CKW : 1)];
       if (blk > 0) {
         const T* c = ck + (size_t)(blk - 1) * CKW * 64 + lane;
 #pragma unroll
@@ -266,76 +263,94 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
     } else {
       const T* c = ck + (size_t)blk * CKW * 64 + lane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        M[r] = c[(size_t)r * 64];
-        D[r] = c[(size_t)(R + r) * 64];
-        Q[r] = c[(size_t)(2 * R + r) * 64];
-      }
-      Plast = c[(size_t)(3 * R) * 64];
-      Xprev = c[(size_t)(3 * R + 1) * 64];
+      for (int w = 0; w < CKW; ++w) cur[w] = c[(size_t)w * 64];
     }
-    T sQ[GK][R], sD[GK][R], sPin[GK];
+    const int tb = blk * CK_STEPS;
+    for (int sub = NSUB - 1; sub >= 0; --sub) {
+      const int t0 = tb + sub * GK;
+      if (t0 > tend) continue;
+      const int cnt = min(GK, tend + 1 - t0);
+      // state before step tb: M = x D^ + M^ and D = 0 make the first step's
+      // D update reproduce D^(tb) exactly
 #pragma unroll
-    for (int s = 0; s < GK; ++s) {
-      if (s < cnt) {
-        const int t = t0 + s;
+      for (int r = 0; r < R; ++r) {
+        M[r] = cur[r];
+        D[r] = 0;
+        Q[r] = cur[R + r];
+      }
+      Plast = cur[2 * R];
+      Xprev = cur[2 * R + 1];
+      for (int t = tb; t < t0; ++t) {  // steps before the register block, not stored
         T pinj, qinj;
         injections<T, ANCH>(t, gp.m, C, pinj, qinj);
         const T Pin = dpp_shr1(pinj, Plast);
         const T Xn = dpp_shr1(qinj, Q[R - 1]);
-        Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], sQ[s], sD[s]);
+        Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
         Xprev = Xn;
-        sPin[s] = Pin;
       }
-    }
+      T sQ[GK][R], sD[GK][R], sPin[GK];
 #pragma unroll
-    for (int s = GK - 1; s >= 0; --s) {
-      if (s < cnt) {
-        const int t = t0 + s;
-        const uint32_t code = cl[t];
-        const bool post = code == SENT_POST;
-        const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
-        const T Pbin = dpp_shl1(Pseed, Ilast);
-        const T Qbin = Eprev;
-        const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
-        const T En = dpp_shl1(einj, EMp[0]);
-        T Qb[R], e[R], sw[R], Iv[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
-        T Ih = sPin[s];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const bool mt = rc[r] == code;
-          e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
-          sw[r] = mt ? T(-1) : smis;
-          Iv[r] = pass[r] ? T(0) : Ih;
-          Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
+      for (int s = 0; s < GK; ++s) {
+        if (s < cnt) {
+          const int t = t0 + s;
+          T pinj, qinj;
+          injections<T, ANCH>(t, gp.m, C, pinj, qinj);
+          const T Pin = dpp_shr1(pinj, Plast);
+          const T Xn = dpp_shr1(qinj, Q[R - 1]);
+          Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], sQ[s], sD[s]);
+          Xprev = Xn;
+          sPin[s] = Pin;
         }
-        T Pb = Pbin;
+      }
 #pragma unroll
-        for (int r = R - 1; r >= 0; --r) {
-          const T gr = r == R - 1 ? gam_last : gam;
-          const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
-          const T Mbar = Pb + Qb[r] + Dbr[r];
-          const T EM = e[r] * Mbar;
-          Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
-          const T u = mu[r] * Pb, v = be[r] * Qb[r];
-          if (r == R - 1) {
-            UL = fmaT(u, Iv[r], UL);
-            VL = fmaT(v, Iv[r], VL);
-            Tl = fmaT(Qb[r], sD[s][r], Tl);
-          } else {
-            U = fmaT(u, Iv[r], U);
-            V = fmaT(v, Iv[r], V);
-            Tg = fmaT(Qb[r], sD[s][r], Tg);
+      for (int s = GK - 1; s >= 0; --s) {
+        if (s < cnt) {
+          const int t = t0 + s;
+          const uint32_t code = cl[t];
+          const bool post = code == SENT_POST;
+          const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
+          const T Pbin = dpp_shl1(Pseed, Ilast);
+          const T Qbin = Eprev;
+          const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
+          const T En = dpp_shl1(einj, EMp[0]);
+          T Qb[R], e[R], sw[R], Iv[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
+          T Ih = sPin[s];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const bool mt = rc[r] == code;
+            e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
+            sw[r] = mt ? T(-1) : smis;
+            Iv[r] = pass[r] ? T(0) : Ih;
+            Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
           }
-          Tx = fmaT(Dbr[r], sD[s][r], Tx);
-          Dbr[r] = Dbar;
-          EMp[r] = EM;
-          Pb = u + v;
+          T Pb = Pbin;
+#pragma unroll
+          for (int r = R - 1; r >= 0; --r) {
+            const T gr = r == R - 1 ? gam_last : gam;
+            const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
+            const T Mbar = Pb + Qb[r] + Dbr[r];
+            const T EM = e[r] * Mbar;
+            Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
+            const T u = mu[r] * Pb, v = be[r] * Qb[r];
+            if (r == R - 1) {
+              UL = fmaT(u, Iv[r], UL);
+              VL = fmaT(v, Iv[r], VL);
+              Tl = fmaT(Qb[r], sD[s][r], Tl);
+            } else {
+              U = fmaT(u, Iv[r], U);
+              V = fmaT(v, Iv[r], V);
+              Tg = fmaT(Qb[r], sD[s][r], Tg);
+            }
+            Tx = fmaT(Dbr[r], sD[s][r], Tx);
+            Dbr[r] = Dbar;
+            EMp[r] = EM;
+            Pb = u + v;
+          }
+          Ilast = Pb;
+          Eprev = En;
         }
-        Ilast = Pb;
-        Eprev = En;
       }
     }
   }
@@ -395,10 +410,13 @@ __device__ __forceinline__ void grad_lds_init(const void* gtab, size_t tab_bytes
 // FP32 adjoint of the units of row class R: the contiguous unit range of the
 // class in the class-major plan, taken from a per-class counter.
 template <bool ANCH, int R>
-__global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
-                                               const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
-                                               size_t ck_stride, const double* __restrict__ weights,
-                                               double* __restrict__ out, double* __restrict__ gpart) {
+__device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ gtab,
+                                            const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
+                                            size_t ck_stride, const double* __restrict__ weights,
+                                            double* __restrict__ out, double* __restrict__ gpart) {
+  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
+  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
+  if (lo >= hi) return;  // no read of this class: the block's partial row stays as it is
   extern __shared__ __align__(16) unsigned char lds[];
   grad_lds_init<float>(gtab, sizeof(QTab<float>), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
@@ -408,8 +426,6 @@ __global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab
   float* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63;
   double acc[2] = {0.0, 0.0};
-  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
-  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
@@ -422,22 +438,61 @@ __global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab
   grad_flush(acc[0], acc[1], lds_de, gpart);
 }
 
+// Classes up to GRAD_W2_MAX rows per lane are compiled for two waves per SIMD
+// (their adjoint is latency-bound at one); wider classes would spill.
+constexpr int GRAD_W2_MAX = 10;
+
+template <bool ANCH, int R>
+__global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
+                                               const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
+                                               size_t ck_stride, const double* __restrict__ weights,
+                                               double* __restrict__ out, double* __restrict__ gpart) {
+  grad32_body<ANCH, R>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
+}
+
+template <bool ANCH, int R>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_grad32w(
+    BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab, const GradTab* __restrict__ gt, int smax,
+    float* __restrict__ ckall, size_t ck_stride, const double* __restrict__ weights, double* __restrict__ out,
+    double* __restrict__ gpart) {
+  grad32_body<ANCH, R>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
+}
+
 // Bins the FP64 gradient list (the pairs the FP32 pass flagged) by rows per
 // lane, rounded up to an even count, so that each FP64 kernel runs with the
 // register footprint of its bin.
 template <bool ANCH>
-__global__ void k_bin64(BatchDev b, PlanDev p, PairRef* __restrict__ bins, int64_t bin_cap) {
+__global__ __launch_bounds__(256) void k_bin64(BatchDev b, PlanDev p, PairRef* __restrict__ bins, int64_t bin_cap) {
+  __shared__ unsigned int cnt[NBIN64];
+  __shared__ unsigned long long base[NBIN64];
   unsigned long long count = p.ctr[CTR_F64];
   if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    const PairRef pr = p.f64_list[i];
-    const int np = (int)(b.read_off[pr.read + 1] - b.read_off[pr.read]) + (ANCH ? 1 : 0);
-    const int rows = (np + GROUP - 1) / GROUP;
-    if (rows > MAX_R32) continue;  // the host rejects such reads before launching
-    const int bin = (rows + 1) / 2 - 1;
-    const unsigned long long k = atomicAdd(&p.ctr[CTR_BIN64 + bin], 1ull);
-    if ((int64_t)k < bin_cap) bins[(size_t)bin * bin_cap + k] = pr;
+  for (unsigned long long i0 = (unsigned long long)blockIdx.x * blockDim.x; i0 < count;
+       i0 += (unsigned long long)gridDim.x * blockDim.x) {
+    if (threadIdx.x < NBIN64) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned long long i = i0 + threadIdx.x;
+    int bin = -1;
+    unsigned int slot = 0;
+    PairRef pr{0, 0, 0};
+    if (i < count) {
+      pr = p.f64_list[i];
+      const int np = (int)(b.read_off[pr.read + 1] - b.read_off[pr.read]) + (ANCH ? 1 : 0);
+      const int rows = (np + GROUP - 1) / GROUP;
+      if (rows <= MAX_R32) {  // the host rejects longer reads before launching
+        bin = (rows + 1) / 2 - 1;
+        slot = atomicAdd(&cnt[bin], 1u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < NBIN64)
+      base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&p.ctr[CTR_BIN64 + threadIdx.x], (unsigned long long)cnt[threadIdx.x]) : 0ull;
+    __syncthreads();
+    if (bin >= 0) {
+      const unsigned long long k = base[bin] + slot;
+      if ((int64_t)k < bin_cap) bins[(size_t)bin * bin_cap + k] = pr;
+    }
+    __syncthreads();
   }
 }
 
@@ -500,15 +555,21 @@ __global__ void k_reset_unit64(unsigned long long* ctr) {
   if (threadIdx.x == 0) ctr[CTR_UNIT64] = 0ull;
 }
 
-// Sum of the per-block partial gradients in block order (deterministic for a
-// given grid).
-__global__ void k_grad_reduce(const double* __restrict__ gpart, int nparts, double* __restrict__ grad) {
-  const int i = threadIdx.x;
-  if (i < NGRAD) {
-    double s = 0.0;
-    for (int k = 0; k < nparts; ++k) s += gpart[(size_t)k * NGRAD + i];
-    grad[i] = s;
+// Sum of the per-block partial gradients, one block per component (a fixed
+// reduction order for a given grid).
+__global__ __launch_bounds__(256) void k_grad_reduce(const double* __restrict__ gpart, int nparts,
+                                                    double* __restrict__ grad) {
+  __shared__ double part[256];
+  const int i = blockIdx.x;  // gradient component
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) v += gpart[(size_t)k * NGRAD + i];
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) grad[i] = part[0];
 }
 
 size_t lds_grad32(int smax) {
@@ -522,15 +583,18 @@ size_t lds_grad64(int smax) {
 
 // checkpoint area of one block, in values of the kernel's precision
 size_t grad_ck_stride(int smax, int fp64) {
-  const int gk = fp64 ? GK64 : GK32;
-  return (size_t)(smax / gk + 2) * CKW_MAX * 64;
+  const int steps = (fp64 ? GK64 : GK32) * NSUB;
+  return (size_t)(smax / steps + 2) * CKW_MAX * 64;
 }
 
 template <bool ANCH>
 static hipError_t occ_grad_r(int r, int fp64, int smax, int* n) {
   switch (fp64 ? r / 2 + 16 : r) {
-#define VD_OCC32(RR) \
-  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax));
+#define VD_OCC32(RR)                                                                                        \
+  case RR:                                                                                                  \
+    return RR <= GRAD_W2_MAX                                                                                \
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32w<ANCH, RR>, 64, lds_grad32(smax)) \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax));
     VD_OCC32(1) VD_OCC32(2) VD_OCC32(3) VD_OCC32(4) VD_OCC32(5) VD_OCC32(6) VD_OCC32(7) VD_OCC32(8)
     VD_OCC32(9) VD_OCC32(10) VD_OCC32(11) VD_OCC32(12) VD_OCC32(13) VD_OCC32(14) VD_OCC32(15) VD_OCC32(16)
 #undef VD_OCC32
@@ -553,9 +617,14 @@ static void launch_grad32_r(int r, dim3 grid, size_t lds, hipStream_t st, const 
                             const QTab<float>* tab32, const GradTab* gt, int smax, float* ck, size_t s32,
                             const double* w, double* out, double* gpart) {
   switch (r) {
-#define VD_L32(RR)                                                                                               \
-  case RR:                                                                                                       \
-    hipLaunchKernelGGL((k_grad32<ANCH, RR>), grid, dim3(64), lds, st, b, p, tab32, gt, smax, ck, s32, w, out, gpart); \
+#define VD_L32(RR)                                                                                          \
+  case RR:                                                                                                  \
+    if (RR <= GRAD_W2_MAX)                                                                                  \
+      hipLaunchKernelGGL((k_grad32w<ANCH, RR>), grid, dim3(64), lds, st, b, p, tab32, gt, smax, ck, s32, w, out, \
+                         gpart);                                                                            \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_grad32<ANCH, RR>), grid, dim3(64), lds, st, b, p, tab32, gt, smax, ck, s32, w, out, \
+                         gpart);                                                                            \
     break;
     VD_L32(1) VD_L32(2) VD_L32(3) VD_L32(4) VD_L32(5) VD_L32(6) VD_L32(7) VD_L32(8)
     VD_L32(9) VD_L32(10) VD_L32(11) VD_L32(12) VD_L32(13) VD_L32(14) VD_L32(15) VD_L32(16)
@@ -612,7 +681,7 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
       launch_grad64_bin<false>(bin, dim3(blocks64[bin]), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64, s64,
                                bins, bin_cap, weights, out, gpart);
   }
-  hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(128), 0, st, (const double*)gpart, max_blocks, grad);
+  hipLaunchKernelGGL(k_grad_reduce, dim3(NGRAD), dim3(256), 0, st, (const double*)gpart, max_blocks, grad);
   if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }

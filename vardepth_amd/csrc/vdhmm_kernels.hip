@@ -287,6 +287,9 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
 template <bool ANCH, int R>
 __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                               int smax, double* __restrict__ out) {
+  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
+  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
+  if (lo >= hi) return;  // no read of this class in the batch
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds);
   uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
@@ -297,8 +300,6 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
-  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);

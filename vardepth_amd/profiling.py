@@ -7,6 +7,15 @@
 
 `pmc` also reports the 256 MiB device copy that ``bench.py --copy-calib``
 runs, so FETCH_SIZE / WRITE_SIZE can be scaled to bytes per dispatch.
+
+    python -m vardepth_amd.profiling sq OUT/run_results.db
+        per kernel (summed over its dispatches): duration, clock from
+        GRBM_GUI_ACTIVE (summed over the 8 XCDs), wave64 VALU instructions per
+        SIMD cycle over the 1024 SIMDs and its fraction of the 0.5 per cycle
+        that FP32 wave64 instructions issue at (2 cycles each)
+    python -m vardepth_amd.profiling traffic FETCH_DB WRITE_DB [CALIB_DB]
+        HBM bytes per dispatch per kernel, each counter scaled by what it
+        reads for the 256 MiB copy (taken from CALIB_DB when given)
 """
 from __future__ import annotations
 
@@ -41,7 +50,71 @@ def copy_calibration(rows, counter: str):
     return best[3], COPY_BYTES / (best[3] * 1024.0)
 
 
+XCDS = 8
+SIMDS = 1024
+VALU_PER_CYCLE = 0.5
+
+
+def by_kernel(db: str):
+    """{kernel name: (dispatches, total duration ns, {counter: total value})}."""
+    con = sqlite3.connect(db)
+    dur = dict(con.execute("select name, sum(end - start) from kernels group by name").fetchall())
+    calls = dict(con.execute("select name, count(*) from kernels group by name").fetchall())
+    out = {}
+    for name, counter, value in con.execute(
+            "select k.name, p.counter_name, sum(p.counter_value) from pmc_events p "
+            "join kernels k on k.dispatch_id = p.dispatch_id group by k.name, p.counter_name").fetchall():
+        out.setdefault(name, (calls[name], dur[name], {}))[2][counter] = value
+    return out
+
+
+def sq_table(db: str):
+    rows = []
+    for name, (n, dur, c) in sorted(by_kernel(db).items(), key=lambda kv: -kv[1][1]):
+        if "SQ_INSTS_VALU" not in c or "GRBM_GUI_ACTIVE" not in c:
+            continue
+        cycles = c["GRBM_GUI_ACTIVE"] / XCDS
+        ipc = c["SQ_INSTS_VALU"] / (SIMDS * cycles) if cycles else 0.0
+        rows.append((name, n, dur / 1e6, cycles / dur if dur else 0.0, c["SQ_INSTS_VALU"], ipc, ipc / VALU_PER_CYCLE,
+                     c.get("SQ_WAVES", 0), c.get("SQ_INSTS_SALU", 0), c.get("SQ_INSTS_LDS", 0)))
+    return rows
+
+
+def traffic_table(fetch_db: str, write_db: str, calib_db: str | None = None):
+    """[(kernel, dispatches, fetch bytes/dispatch, write bytes/dispatch)] and the two factors."""
+    fr, wr = pmc(fetch_db), pmc(write_db)
+    cf = copy_calibration(pmc(calib_db) if calib_db else fr, "FETCH_SIZE")[1]
+    cw = copy_calibration(pmc(calib_db) if calib_db else wr, "WRITE_SIZE")[1]
+    if cf is None or cw is None:
+        raise SystemExit("no 256 MiB copy in the calibration database (run bench.py with --copy-calib)")
+    agg = {}
+    for rows, key, f in ((fr, "FETCH_SIZE", cf), (wr, "WRITE_SIZE", cw)):
+        for _, name, counter, value in rows:
+            if counter == key and "copy" not in name.lower():
+                e = agg.setdefault(name, {"FETCH_SIZE": [], "WRITE_SIZE": []})
+                e[key].append(value * 1024.0 * f)
+    out = []
+    for name, e in agg.items():
+        nf, nw = len(e["FETCH_SIZE"]), len(e["WRITE_SIZE"])
+        out.append((name, max(nf, nw), sum(e["FETCH_SIZE"]) / max(nf, 1), sum(e["WRITE_SIZE"]) / max(nw, 1)))
+    return sorted(out, key=lambda r: -(r[2] + r[3])), cf, cw
+
+
 def main(argv):
+    if len(argv) >= 2 and argv[0] == "sq":
+        print("%-60s %5s %10s %6s %12s %8s %6s %6s" % ("kernel", "calls", "ms", "GHz", "VALU", "per-cyc", "util",
+                                                       "waves"))
+        for r in sq_table(argv[1]):
+            print("%-60s %5d %10.3f %6.2f %12.4g %8.3f %6.3f %6d" % (r[0][:60], r[1], r[2], r[3], r[4], r[5], r[6],
+                                                                   r[7]))
+        return 0
+    if len(argv) >= 3 and argv[0] == "traffic":
+        rows, cf, cw = traffic_table(argv[1], argv[2], argv[3] if len(argv) > 3 else None)
+        print("calibration: FETCH_SIZE x%.3f, WRITE_SIZE x%.3f (256 MiB copy)" % (cf, cw))
+        print("%-60s %5s %14s %14s" % ("kernel", "calls", "fetch B/disp", "write B/disp"))
+        for r in rows:
+            print("%-60s %5d %14.4g %14.4g" % (r[0][:60], r[1], r[2], r[3]))
+        return 0
     if len(argv) < 2 or argv[0] not in ("kernels", "pmc"):
         print(__doc__)
         return 2
