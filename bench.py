@@ -189,7 +189,7 @@ def plain_baseline(batch, mode: int, pairs: int):
             "kind": "one kernel launch per pair, one thread per read row, FP32, inputs resident in HBM"}
 
 
-def grad_bench(regions: int, steps: int, mode: int):
+def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     """Forward and forward+adjoint GCUPS on the HMM-sampled workload (all pairs,
     unit weights: gradient of sum ln L wrt gap, ext and the 64-bin error table)."""
     import torch
@@ -197,6 +197,11 @@ def grad_bench(regions: int, steps: int, mode: int):
     b = workloads.hmm_sampled_regions(seed=41004, regions=regions)
     cells = b.cells()
     dev = engine.DeviceBatch(b)
+    tune_f = tune_g = None
+    if tune:
+        from vardepth_amd import autotune
+        tune_f = autotune.tune_forward(dev, mode=mode)
+        tune_g = autotune.tune_grad(dev, mode=mode)
     dev.forward(mode=mode)
     dev.grad(mode=mode)
     torch.cuda.synchronize()
@@ -217,7 +222,8 @@ def grad_bench(regions: int, steps: int, mode: int):
     return {"workload": "hmm-sampled (%d regions x 100 reads x 20 haplotypes, 150 bp, seed 41004)" % regions,
             "pairs": b.npairs, "cells": cells, "fwd_ms": round(fwd_ms, 3), "adjoint_ms": round(grad_ms, 3),
             "adjoint_kernel_ms": round(k32, 3), "fwd_gcups": round(fwd_g, 3), "fwd_bwd_gcups": round(fb_g, 3),
-            "ratio": round(fb_g / fwd_g, 4), "flagged": int(sts[-1]["flagged"])}
+            "ratio": round(fb_g / fwd_g, 4), "flagged": int(sts[-1]["flagged"]),
+            "autotune": {"forward": tune_f, "adjoint": tune_g}}
 
 
 def main():
@@ -232,6 +238,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch choices)")
     ap.add_argument("--no-forward", action="store_true",
                     help="skip the forward workload (profiling the fwd+bwd workload alone); prints no JSON line")
     ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
@@ -249,7 +256,8 @@ def main():
     from vardepth_amd import engine
 
     if args.no_forward:
-        print(json.dumps({"fwd_bwd": grad_bench(args.grad_regions, args.grad_steps, args.mode)}), flush=True)
+        print(json.dumps({"fwd_bwd": grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)}),
+              flush=True)
         return
 
     t0 = time.perf_counter()
@@ -261,6 +269,13 @@ def main():
         wname, batch.nregions, batch.npairs, cells, gen_s), file=sys.stderr, flush=True)
 
     dev = engine.DeviceBatch(batch)
+    tune = None
+    if not args.no_tune:
+        from vardepth_amd import autotune
+        t0 = time.perf_counter()
+        tune = autotune.tune_forward(dev, mode=args.mode, precision=args.precision)
+        tune["seconds"] = round(time.perf_counter() - t0, 2)
+        print("autotune forward: %s" % json.dumps(tune), file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         dev.forward(mode=args.mode, precision=args.precision, stats=True)
     torch.cuda.synchronize()
@@ -301,7 +316,7 @@ def main():
 
     fwd_bwd = None
     if args.grad_regions > 0:
-        fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode)
+        fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)
         print("fwd+bwd: %s" % json.dumps(fwd_bwd), file=sys.stderr, flush=True)
 
     if args.copy_calib:
@@ -326,7 +341,8 @@ def main():
                    "gap": batch.gap, "ext": batch.ext,
                    "flagged_pairs": last["flagged"], "fp64_pairs": last["fp64_pairs"],
                    "stage_ms": {"plan": round(last["plan_ms"], 3), "fwd32": round(last["fwd32_ms"], 3),
-                                "fwd64": round(last["fwd64_ms"], 3)}},
+                                "fwd64": round(last["fwd64_ms"], 3)},
+                   "autotune": tune},
         "roofline": {"bound": "valu", "kernel": "k_fwd32", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                      "traffic": measured_traffic(args), "traffic_unit": "bytes/launch (rocprofv3 PMC)",

@@ -155,6 +155,30 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
                    int max_hap_len, int max_read_len, const double *dev_weights, double *dev_out,
                    double *dev_grad, void *stream, vd_stats *stats);
 
+/* ---- tuning (no counterpart in the reference) ---------------------------- */
+
+/* Launch choices per rows-per-lane class r = 1..16 (index 0 unused), picked by
+ * vardepth_amd/autotune.py from timed sweeps:
+ *   fwd_unroll          anti-diagonals per iteration of the forward main loop (2 or 4)
+ *   fwd_blocks_per_cu   persistent forward grid per CU (0 = the occupancy limit)
+ *   grad_variant        adjoint register block: 0 = 2 steps (two waves per SIMD
+ *                       where it fits), 1 = 4 steps
+ *   grad_blocks_per_cu  persistent adjoint grid per CU (0 = the occupancy limit)
+ * Defaults: 2, 0, 0, 0.  Values never change results beyond rounding order. */
+typedef struct vd_tuning {
+  int fwd_unroll[17];
+  int fwd_blocks_per_cu[17];
+  int grad_variant[17];
+  int grad_blocks_per_cu[17];
+} vd_tuning;
+
+int vd_tuning_get(vd_tuning *t);
+int vd_tuning_set(const vd_tuning *t);
+/* When on, calls made with a stats pointer also time each class kernel;
+ * vd_class_ms returns the last such times (17 each, index = class). */
+void vd_class_timing(int on);
+int vd_class_ms(double *fwd_ms, double *grad_ms);
+
 /* ---- baseline ----------------------------------------------------------- */
 
 /* The plain GPU forward: the first max_pairs pairs of the batch (region

@@ -150,3 +150,32 @@ def test_plain_baseline_matches():
     gb = region_batch_from_pairs([grid[i] for i in short])
     out1, _ = engine.forward_plain(gb, mode=1)
     close(out1, G["grid_dense_m1"][short], TOL_MIXED)
+
+
+def test_autotune_keeps_results():
+    """Every launch choice gives the same likelihoods and gradients."""
+    from vardepth_amd import autotune
+    b = workloads.uneven_regions(seed=9, regions=24)
+    dev = engine.DeviceBatch(b)
+    base = autotune.get()
+    dev.forward(mode=0)
+    ref = dev.result()
+    g_ref = dev.grad(mode=0).cpu().numpy()
+    rep = autotune.tune_forward(dev, mode=0, reps=1)
+    rep2 = autotune.tune_grad(dev, mode=0, reps=1)
+    assert rep["choice"] and rep2["choice"]
+    for cand in autotune.FWD_CANDIDATES:
+        t = autotune.get()
+        for r in range(1, 17):
+            t.fwd_unroll[r], t.fwd_blocks_per_cu[r] = cand
+        autotune.set_tuning(t)
+        dev.forward(mode=0)
+        np.testing.assert_array_equal(dev.result(), ref)
+    for cand in autotune.GRAD_CANDIDATES:
+        t = autotune.get()
+        for r in range(1, 17):
+            t.grad_variant[r], t.grad_blocks_per_cu[r] = cand
+        autotune.set_tuning(t)
+        g = dev.grad(mode=0).cpu().numpy()
+        np.testing.assert_allclose(g, g_ref, rtol=1e-12, atol=1e-12)
+    autotune.set_tuning(base)

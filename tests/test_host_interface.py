@@ -258,3 +258,22 @@ def test_grad_rejects_long_reads_before_device():
         engine.grad(b1, mode=1)
     with pytest.raises(ValueError, match="one entry per"):
         engine.grad(b1, weights=[1.0, 2.0])
+
+
+def test_tuning_get_set_validation():
+    from vardepth_amd import _lib, autotune
+    t = autotune.get()
+    assert list(t.fwd_unroll)[1:] == [2] * 16 or all(v in (2, 4) for v in list(t.fwd_unroll)[1:])
+    bad = _lib.VdTuning.from_buffer_copy(t)
+    bad.fwd_unroll[5] = 3
+    with pytest.raises(_lib.VdhmmError, match="fwd_unroll"):
+        autotune.set_tuning(bad)
+    bad = _lib.VdTuning.from_buffer_copy(t)
+    bad.grad_variant[7] = 2
+    with pytest.raises(_lib.VdhmmError, match="grad_variant"):
+        autotune.set_tuning(bad)
+    good = _lib.VdTuning.from_buffer_copy(t)
+    good.fwd_unroll[10] = 4
+    autotune.set_tuning(good)
+    assert autotune.get().fwd_unroll[10] == 4
+    autotune.set_tuning(t)

@@ -28,8 +28,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch",
            "read_haplotype_batch_mode", "wide_capability", "anchored_pairhmm",
            "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
-           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_last_error",
-           "vd_device")
+           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_tuning_get",
+           "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_last_error", "vd_device")
 
 
 class VdParams(ctypes.Structure):
@@ -52,6 +52,14 @@ class VdStats(ctypes.Structure):
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class VdTuning(ctypes.Structure):
+    _fields_ = [("fwd_unroll", ctypes.c_int * 17), ("fwd_blocks_per_cu", ctypes.c_int * 17),
+                ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17)]
+
+    def as_dict(self):
+        return {name: list(getattr(self, name)) for name, _ in self._fields_}
 
 
 class VdhmmError(RuntimeError):
@@ -104,6 +112,14 @@ def lib() -> ctypes.CDLL:
     L.vd_forward_plain.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int, ctypes.c_int64,
                                    _dp, _dp]
     L.vd_forward_plain.restype = ctypes.c_int
+    L.vd_tuning_get.argtypes = [ctypes.POINTER(VdTuning)]
+    L.vd_tuning_get.restype = ctypes.c_int
+    L.vd_tuning_set.argtypes = [ctypes.POINTER(VdTuning)]
+    L.vd_tuning_set.restype = ctypes.c_int
+    L.vd_class_timing.argtypes = [ctypes.c_int]
+    L.vd_class_timing.restype = None
+    L.vd_class_ms.argtypes = [_dp, _dp]
+    L.vd_class_ms.restype = ctypes.c_int
     L.vd_last_error.argtypes = []
     L.vd_last_error.restype = ctypes.c_char_p
     L.vd_device.argtypes = []

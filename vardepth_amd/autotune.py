@@ -1,0 +1,147 @@
+"""Per-class launch choices picked from timed sweeps on the caller's own batch.
+
+The engine runs one persistent kernel per rows-per-lane class (1..16), so one
+timed pass of a candidate setting gives every class's kernel time at once
+(per-class HIP events, ``vd_class_timing``); a sweep costs candidates x reps
+passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
+
+* forward: anti-diagonals per main-loop iteration (2 or 4) x persistent grid
+  per CU (the occupancy limit, or 2 or 3 blocks per CU);
+* adjoint: register block of 2 steps (two waves per SIMD where that fits) or
+  4 steps.
+
+None of them changes the arithmetic of a pair, so results are identical for
+every choice; only the time differs.  ``tune_forward`` / ``tune_grad`` apply
+the per-class best and return a report; ``save`` / ``load`` keep a choice
+in a JSON file keyed by device and mode.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+
+FWD_CANDIDATES = [(2, 0), (4, 0), (2, 3), (4, 3), (2, 2)]   # (unroll, blocks per CU; 0 = occupancy limit)
+GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
+
+
+def get() -> _lib.VdTuning:
+    t = _lib.VdTuning()
+    _lib.check(_lib.lib().vd_tuning_get(ctypes.byref(t)), "vd_tuning_get")
+    return t
+
+
+def set_tuning(t: _lib.VdTuning):
+    _lib.check(_lib.lib().vd_tuning_set(ctypes.byref(t)), "vd_tuning_set")
+
+
+def class_ms():
+    f = (ctypes.c_double * 17)()
+    g = (ctypes.c_double * 17)()
+    _lib.lib().vd_class_ms(f, g)
+    return np.array(f[:]), np.array(g[:])
+
+
+def _sweep(run, candidates, apply, which: int, reps: int):
+    """Per candidate: the minimum over reps of each class's kernel time."""
+    L = _lib.lib()
+    base = get()
+    times = {}
+    L.vd_class_timing(1)
+    try:
+        for cand in candidates:
+            t = _lib.VdTuning.from_buffer_copy(base)
+            for r in range(1, 17):
+                apply(t, r, cand)
+            set_tuning(t)
+            best = np.full(17, np.inf)
+            for _ in range(reps):
+                run()
+                best = np.minimum(best, class_ms()[which])
+            times[cand] = best
+    finally:
+        L.vd_class_timing(0)
+        set_tuning(base)
+    return times
+
+
+def _pick(times, candidates, active):
+    choice = {}
+    for r in range(1, 17):
+        if not active[r]:
+            continue
+        choice[r] = min(candidates, key=lambda c: times[c][r])
+    return choice
+
+
+def _apply_fwd(t, r, cand):
+    t.fwd_unroll[r], t.fwd_blocks_per_cu[r] = cand
+
+
+def _apply_grad(t, r, cand):
+    t.grad_variant[r], t.grad_blocks_per_cu[r] = cand
+
+
+def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CANDIDATES):
+    """Sweep the forward choices on a DeviceBatch and apply the best per class."""
+    def run():
+        dev_batch.forward(mode=mode, precision=precision, stats=True)
+
+    run()  # warm: buffers and tables
+    times = _sweep(run, candidates, _apply_fwd, 0, reps)
+    default = times[candidates[0]]
+    active = default > 0.02  # classes with work (an empty class kernel exits at once)
+    choice = _pick(times, candidates, active)
+    t = get()
+    for r, c in choice.items():
+        _apply_fwd(t, r, c)
+    set_tuning(t)
+    tuned = sum(times[c][r] for r, c in choice.items())
+    return {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
+            "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+
+
+def tune_grad(dev_batch, mode=0, reps=1, candidates=GRAD_CANDIDATES):
+    """Sweep the adjoint choices on a DeviceBatch and apply the best per class."""
+    def run():
+        dev_batch.grad(mode=mode, stats=True)
+
+    run()
+    times = _sweep(run, candidates, _apply_grad, 1, reps)
+    default = times[candidates[0]]
+    active = default > 0.02
+    choice = _pick(times, candidates, active)
+    t = get()
+    for r, c in choice.items():
+        _apply_grad(t, r, c)
+    set_tuning(t)
+    tuned = sum(times[c][r] for r, c in choice.items())
+    return {"kind": "adjoint", "choice": {r: list(c) for r, c in choice.items()},
+            "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+
+
+def save(path, key: str):
+    p = Path(path)
+    data = json.loads(p.read_text()) if p.exists() else {}
+    data[key] = get().as_dict()
+    p.write_text(json.dumps(data, indent=1, sort_keys=True))
+
+
+def load(path, key: str) -> bool:
+    p = Path(path)
+    if not p.exists():
+        return False
+    data = json.loads(p.read_text())
+    if key not in data:
+        return False
+    t = _lib.VdTuning()
+    for name, _ in t._fields_:
+        arr = getattr(t, name)
+        for i, v in enumerate(data[key][name]):
+            arr[i] = int(v)
+    set_tuning(t)
+    return True

@@ -72,9 +72,14 @@ struct Engine {
   DevBuf ck32, ck64, gpart, bins64, in_w, in_grad;
   vd_params params_on_device;
   bool have_params = false;
-  int occ32[2][17] = {}, occ64[2] = {0, 0};
+  int occ32[2][17][2] = {}, occ64[2] = {0, 0};  // [anchored][class][unroll 2 / 4]
   int occ_smax = -1;
-  int gocc32[2][17] = {}, gocc64[2][8] = {};
+  int gocc32[2][17][2] = {}, gocc64[2][8] = {};  // [anchored][class][variant]
+  vd_tuning tune;
+  bool tune_init = false;
+  bool class_timing = false;
+  hipEvent_t ev_cls[17] = {};
+  double fwd_cls_ms[17] = {}, grad_cls_ms[17] = {};
   int gocc_smax = -1;
 };
 
@@ -95,6 +100,7 @@ int engine_init() {
   g.cus = prop.multiProcessorCount;
   HIPCHK(hipStreamCreateWithFlags(&g.own_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
+  for (int i = 0; i < 17; ++i) HIPCHK(hipEventCreate(&g.ev_cls[i]));
   g.dev = dev;
   return 0;
 }
@@ -149,6 +155,17 @@ int upload_params(const vd_params& p, hipStream_t st) {
   return 0;
 }
 
+void tune_defaults() {
+  if (g.tune_init) return;
+  for (int r = 0; r < 17; ++r) {
+    g.tune.fwd_unroll[r] = 2;
+    g.tune.fwd_blocks_per_cu[r] = 0;
+    g.tune.grad_variant[r] = 0;
+    g.tune.grad_blocks_per_cu[r] = 0;
+  }
+  g.tune_init = true;
+}
+
 int plan_buffers(const vd_batch* db, int64_t npairs) {
   const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
   const int64_t nb = db->nregions * vd::NCLS;
@@ -192,20 +209,38 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.occ_smax != smax) {
     for (int a = 0; a < 2; ++a) {
-      for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::occupancy(a, r, smax, &g.occ32[a][r]));
-      HIPCHK(vd::occupancy(a, 0, smax, &g.occ64[a]));
+      for (int r = 1; r <= vd::MAX_R32; ++r) {
+        HIPCHK(vd::occupancy(a, r, 2, smax, &g.occ32[a][r][0]));
+        HIPCHK(vd::occupancy(a, r, 4, smax, &g.occ32[a][r][1]));
+      }
+      HIPCHK(vd::occupancy(a, 0, 2, smax, &g.occ64[a]));
     }
     g.occ_smax = smax;
   }
-  int blocks32[17] = {0};
-  for (int r = 1; r <= vd::MAX_R32; ++r) blocks32[r] = std::max(1, g.cus * std::max(1, g.occ32[anchored][r]));
+  tune_defaults();
+  int blocks32[17] = {0}, unroll32[17] = {0};
+  for (int r = 1; r <= vd::MAX_R32; ++r) {
+    unroll32[r] = g.tune.fwd_unroll[r] == 4 ? 4 : 2;
+    const int occ = std::max(1, g.occ32[anchored][r][unroll32[r] == 4 ? 1 : 0]);
+    const int want = g.tune.fwd_blocks_per_cu[r];
+    blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
+  }
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   const vd::BatchDev b = batch_dev(db);
   const vd::PlanDev p = plan_dev(npairs, 1);
+  const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
-                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks64,
-                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr));
+                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, unroll32, blocks64,
+                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr, timing ? g.ev_cls : nullptr));
+  if (timing && !force64) {
+    HIPCHK(hipEventSynchronize(g.ev[3]));
+    for (int r = 1; r <= vd::MAX_R32; ++r) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, g.ev_cls[r], g.ev_cls[r - 1]));
+      g.fwd_cls_ms[r] = ms;
+    }
+  }
   if (stats) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     unsigned long long c[vd::CTR_COUNT];
@@ -359,16 +394,21 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.gocc_smax != smax) {
     for (int a = 0; a < 2; ++a) {
-      for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::grad_occupancy(a, r, 0, smax, &g.gocc32[a][r]));
-      for (int k = 0; k < 8; ++k) HIPCHK(vd::grad_occupancy(a, 2 * (k + 1), 1, smax, &g.gocc64[a][k]));
+      for (int r = 1; r <= vd::MAX_R32; ++r)
+        for (int v = 0; v < 2; ++v) HIPCHK(vd::grad_occupancy(a, r, 0, v, smax, &g.gocc32[a][r][v]));
+      for (int k = 0; k < 8; ++k) HIPCHK(vd::grad_occupancy(a, 2 * (k + 1), 1, 0, smax, &g.gocc64[a][k]));
     }
     g.gocc_smax = smax;
   }
+  tune_defaults();
   // per-class persistent grids; the checkpoint and partial-gradient areas are
   // sized for the widest of them
-  int blocks32[17] = {0}, blocks64[8] = {0}, max_blocks = 1;
+  int blocks32[17] = {0}, variant32[17] = {0}, blocks64[8] = {0}, max_blocks = 1;
   for (int r = 1; r <= vd::MAX_R32; ++r) {
-    blocks32[r] = std::max(1, g.cus * std::max(1, g.gocc32[anchored][r]));
+    variant32[r] = g.tune.grad_variant[r] == 1 ? 1 : 0;
+    const int occ = std::max(1, g.gocc32[anchored][r][variant32[r]]);
+    const int want = g.tune.grad_blocks_per_cu[r];
+    blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
     max_blocks = std::max(max_blocks, blocks32[r]);
   }
   int max64 = 1;
@@ -385,9 +425,18 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   const vd::BatchDev b = batch_dev(db);
   const vd::PlanDev p = plan_dev(npairs, 1);
   HIPCHK(vd::launch_grad(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(), g.tab64.as<vd::QTab<double>>(),
-                         g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, blocks64, max_blocks,
+                         g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, variant32, blocks64, max_blocks,
                          g.ck32.as<float>(), g.ck64.as<double>(), g.bins64.as<vd::PairRef>(), bin_cap, dweights,
-                         dout, g.gpart.as<double>(), dgrad, st, stats ? g.ev : nullptr));
+                         dout, g.gpart.as<double>(), dgrad, st, stats ? g.ev : nullptr,
+                         stats && g.class_timing ? g.ev_cls : nullptr));
+  if (stats && g.class_timing) {
+    HIPCHK(hipEventSynchronize(g.ev[3]));
+    for (int r = 1; r <= vd::MAX_R32; ++r) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, g.ev_cls[r], g.ev_cls[r - 1]));
+      g.grad_cls_ms[r] = ms;
+    }
+  }
   if (stats) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     unsigned long long c[vd::CTR_COUNT];
@@ -686,6 +735,41 @@ int vd_forward_plain(const vd_batch* batch, const vd_params* params, int mode, i
   if (!batch || !params || !out) return fail("vdhmm: null argument");
   std::lock_guard<std::mutex> lk(g_mu);
   return plain_host_locked(batch, params, mode, max_pairs, out, gpu_ms);
+}
+
+int vd_tuning_get(vd_tuning* t) {
+  if (!t) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  tune_defaults();
+  *t = g.tune;
+  return 0;
+}
+
+int vd_tuning_set(const vd_tuning* t) {
+  if (!t) return fail("vdhmm: null argument");
+  for (int r = 1; r < 17; ++r) {
+    if (t->fwd_unroll[r] != 2 && t->fwd_unroll[r] != 4) return fail("vdhmm: fwd_unroll must be 2 or 4");
+    if (t->grad_variant[r] != 0 && t->grad_variant[r] != 1) return fail("vdhmm: grad_variant must be 0 or 1");
+    if (t->fwd_blocks_per_cu[r] < 0 || t->grad_blocks_per_cu[r] < 0) return fail("vdhmm: negative grid size");
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.tune = *t;
+  g.tune_init = true;
+  return 0;
+}
+
+void vd_class_timing(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g.class_timing = on != 0;
+}
+
+int vd_class_ms(double* fwd_ms, double* grad_ms) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int r = 0; r < 17; ++r) {
+    if (fwd_ms) fwd_ms[r] = g.fwd_cls_ms[r];
+    if (grad_ms) grad_ms[r] = g.grad_cls_ms[r];
+  }
+  return 0;
 }
 
 const char* vd_last_error(void) { return g_err.c_str(); }

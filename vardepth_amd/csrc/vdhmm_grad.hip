@@ -46,7 +46,8 @@
 
 namespace vd {
 
-constexpr int GK32 = 2;      // register block (anti-diagonal steps), FP32
+constexpr int GK32 = 2;      // register block (anti-diagonal steps), FP32 (default variant)
+constexpr int GK32B = 4;     // the alternative FP32 variant the autotuner may pick
 constexpr int GK64 = 2;      // FP64 (twice the registers per value)
 constexpr int NSUB = 2;      // register blocks per checkpoint
 constexpr int CKW_MAX = 2 * MAX_R32 + 2;
@@ -409,7 +410,7 @@ __device__ __forceinline__ void grad_lds_init(const void* gtab, size_t tab_bytes
 
 // FP32 adjoint of the units of row class R: the contiguous unit range of the
 // class in the class-major plan, taken from a per-class counter.
-template <bool ANCH, int R>
+template <bool ANCH, int R, int GK>
 __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ gtab,
                                             const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
                                             size_t ck_stride, const double* __restrict__ weights,
@@ -433,13 +434,16 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
     if (u >= hi) break;
     GroupPair gp;
     if (decode_unit32(b, p, u, gp, out) != R) continue;
-    unit_grad<float, R, ANCH, GK32>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false);
+    unit_grad<float, R, ANCH, GK>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false);
   }
   grad_flush(acc[0], acc[1], lds_de, gpart);
 }
 
-// Classes up to GRAD_W2_MAX rows per lane are compiled for two waves per SIMD
-// (their adjoint is latency-bound at one); wider classes would spill.
+// Two variants per class: k_grad32w (register blocks of GK32 = 2 steps,
+// compiled for two waves per SIMD, for classes up to GRAD_W2_MAX rows per lane
+// where that fits without heavy spills) and k_grad32 (GK32B = 4 steps, the
+// occupancy the registers allow).  The autotuner picks one per class; the
+// default is k_grad32w where it exists.
 constexpr int GRAD_W2_MAX = 10;
 
 template <bool ANCH, int R>
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(64) void k_grad32(BatchDev b, PlanDev p, const QTab
                                                const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
                                                size_t ck_stride, const double* __restrict__ weights,
                                                double* __restrict__ out, double* __restrict__ gpart) {
-  grad32_body<ANCH, R>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
+  grad32_body<ANCH, R, GK32B>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
 }
 
 template <bool ANCH, int R>
@@ -455,7 +459,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab, const GradTab* __restrict__ gt, int smax,
     float* __restrict__ ckall, size_t ck_stride, const double* __restrict__ weights, double* __restrict__ out,
     double* __restrict__ gpart) {
-  grad32_body<ANCH, R>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
+  grad32_body<ANCH, R, GK32>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
 }
 
 // Bins the FP64 gradient list (the pairs the FP32 pass flagged) by rows per
@@ -581,18 +585,19 @@ size_t lds_grad64(int smax) {
          (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
-// checkpoint area of one block, in values of the kernel's precision
+// checkpoint area of one block, in values of the kernel's precision (sized for
+// the variant with the shortest checkpoint interval)
 size_t grad_ck_stride(int smax, int fp64) {
-  const int steps = (fp64 ? GK64 : GK32) * NSUB;
+  const int steps = (fp64 ? GK64 : (GK32 < GK32B ? GK32 : GK32B)) * NSUB;
   return (size_t)(smax / steps + 2) * CKW_MAX * 64;
 }
 
 template <bool ANCH>
-static hipError_t occ_grad_r(int r, int fp64, int smax, int* n) {
+static hipError_t occ_grad_r(int r, int fp64, int variant, int smax, int* n) {
   switch (fp64 ? r / 2 + 16 : r) {
 #define VD_OCC32(RR)                                                                                        \
   case RR:                                                                                                  \
-    return RR <= GRAD_W2_MAX                                                                                \
+    return (RR <= GRAD_W2_MAX && variant == 0)                                                              \
                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32w<ANCH, RR>, 64, lds_grad32(smax)) \
                : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax));
     VD_OCC32(1) VD_OCC32(2) VD_OCC32(3) VD_OCC32(4) VD_OCC32(5) VD_OCC32(6) VD_OCC32(7) VD_OCC32(8)
@@ -606,20 +611,23 @@ static hipError_t occ_grad_r(int r, int fp64, int smax, int* n) {
   }
 }
 
-// blocks per CU of the FP32 kernel of class r (1..16), or of the FP64 kernel
-// of the bin with r rows per lane (r = 2, 4, ..., 16) when fp64
-hipError_t grad_occupancy(int anchored, int r, int fp64, int smax, int* blocks_per_cu) {
-  return anchored ? occ_grad_r<true>(r, fp64, smax, blocks_per_cu) : occ_grad_r<false>(r, fp64, smax, blocks_per_cu);
+// blocks per CU of the FP32 kernel of class r (1..16) in variant `variant`
+// (0: two-step register blocks, 1: four-step), or of the FP64 kernel of the bin
+// with r rows per lane (r = 2, 4, ..., 16) when fp64
+hipError_t grad_occupancy(int anchored, int r, int fp64, int variant, int smax, int* blocks_per_cu) {
+  return anchored ? occ_grad_r<true>(r, fp64, variant, smax, blocks_per_cu)
+                  : occ_grad_r<false>(r, fp64, variant, smax, blocks_per_cu);
 }
 
 template <bool ANCH>
-static void launch_grad32_r(int r, dim3 grid, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
+static void launch_grad32_r(int r, int variant, dim3 grid, size_t lds, hipStream_t st, const BatchDev& b,
+                            const PlanDev& p,
                             const QTab<float>* tab32, const GradTab* gt, int smax, float* ck, size_t s32,
                             const double* w, double* out, double* gpart) {
   switch (r) {
 #define VD_L32(RR)                                                                                          \
   case RR:                                                                                                  \
-    if (RR <= GRAD_W2_MAX)                                                                                  \
+    if (RR <= GRAD_W2_MAX && variant == 0)                                                                  \
       hipLaunchKernelGGL((k_grad32w<ANCH, RR>), grid, dim3(64), lds, st, b, p, tab32, gt, smax, ck, s32, w, out, \
                          gpart);                                                                            \
     else                                                                                                    \
@@ -651,22 +659,24 @@ static void launch_grad64_bin(int bin, dim3 grid, size_t lds, hipStream_t st, co
 
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                        const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
-                       const int* blocks32, const int* blocks64, int max_blocks, float* ck32, double* ck64,
-                       PairRef* bins, int64_t bin_cap, const double* weights, double* out, double* gpart,
-                       double* grad, hipStream_t st, hipEvent_t* ev) {
+                       const int* blocks32, const int* variant32, const int* blocks64, int max_blocks, float* ck32,
+                       double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
+                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls) {
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, 0, out, st);
   hipLaunchKernelGGL(k_zero_f64, dim3(64), dim3(256), 0, st, gpart, (int64_t)max_blocks * NGRAD);
   if (ev) (void)hipEventRecord(ev[1], st);
   const size_t s32 = grad_ck_stride(smax, 0), s64 = grad_ck_stride(smax, 1);
   for (int r = MAX_R32; r >= 1; --r) {
+    if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
     if (anchored)
-      launch_grad32_r<true>(r, dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32, s32, weights,
-                            out, gpart);
+      launch_grad32_r<true>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
+                            s32, weights, out, gpart);
     else
-      launch_grad32_r<false>(r, dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32, s32, weights,
-                             out, gpart);
+      launch_grad32_r<false>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
+                             s32, weights, out, gpart);
   }
+  if (ev_cls) (void)hipEventRecord(ev_cls[0], st);
   if (ev) (void)hipEventRecord(ev[2], st);
   if (anchored)
     hipLaunchKernelGGL(k_bin64<true>, dim3(256), dim3(256), 0, st, b, p, bins, bin_cap);

@@ -117,21 +117,22 @@ struct LaunchCfg {
 namespace vd {
 size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
-hipError_t occupancy(int anchored, int r, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
+hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
-                           int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev);
+                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
+                           hipEvent_t* ev, hipEvent_t* ev_cls);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
 size_t lds_grad32(int smax);
 size_t lds_grad64(int smax);
 size_t grad_ck_stride(int smax, int fp64);
-hipError_t grad_occupancy(int anchored, int r, int fp64, int smax, int* blocks_per_cu);
+hipError_t grad_occupancy(int anchored, int r, int fp64, int variant, int smax, int* blocks_per_cu);
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                        const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
-                       const int* blocks32, const int* blocks64, int max_blocks, float* ck32, double* ck64,
-                       PairRef* bins, int64_t bin_cap, const double* weights, double* out, double* gpart,
-                       double* grad, hipStream_t st, hipEvent_t* ev);
+                       const int* blocks32, const int* variant32, const int* blocks64, int max_blocks, float* ck32,
+                       double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
+                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls);
 }  // namespace vd

@@ -161,7 +161,7 @@ __global__ void k_unit_map(const int64_t* __restrict__ scanned, int64_t nb, cons
   }
 }
 
-template <int R, bool ANCH>
+template <int R, bool ANCH, int U>
 __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
                                     uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ out) {
   const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
@@ -218,8 +218,9 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
     if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
   }
   const float inj = ANCH ? 0.f : C;
-  // One step at a time; the loops are unrolled by two so that the loop-carried
-  // M/D/Q registers return to the same physical registers without copies.
+  // One step at a time; the loops are unrolled by an even count so that the
+  // loop-carried M/D/Q registers return to the same physical registers
+  // without copies.
   auto main_step = [&](int tt) {
     const uint32_t code = cl[tt];
     float Pin = dpp_shr1(inj, Plast);
@@ -243,12 +244,14 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
       acc += Plast;
     }
   };
+  // U anti-diagonals per iteration of the main loop (U even keeps the
+  // loop-carried registers in place); the autotuner picks U per class
   int t = 1;
-  for (; t + 1 < tmain; t += 2) {
-    main_step(t);
-    main_step(t + 1);
+  for (; t + U - 1 < tmain; t += U) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) main_step(t + k);
   }
-  if (t < tmain) main_step(t++);
+  for (; t < tmain; ++t) main_step(t);
   for (; t + 1 <= tend; t += 2) {
     tail_step(t);
     tail_step(t + 1);
@@ -284,7 +287,7 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
 // FP32 forward of the units of row class R (one kernel per class, so that each
 // runs with its own register footprint): the class's contiguous unit range in
 // the class-major plan, taken from a per-class counter.
-template <bool ANCH, int R>
+template <bool ANCH, int R, int U>
 __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                               int smax, double* __restrict__ out) {
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
     if (u >= hi) break;
     GroupPair gp;
     if (decode_unit32(b, p, u, gp, out) != R) continue;
-    unit32<R, ANCH>(b, p, tab, codes, smax, gp, out);
+    unit32<R, ANCH, U>(b, p, tab, codes, smax, gp, out);
   }
 }
 
@@ -530,11 +533,11 @@ size_t lds_fwd64(int smax) {
   return ((sizeof(QTab<double>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
-template <bool ANCH>
+template <bool ANCH, int U>
 static hipError_t occ_fwd32(int r, int smax, int* n) {
   switch (r) {
 #define VD_OCC(RR) \
-  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32<ANCH, RR>, 64, lds_fwd32(smax));
+  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32<ANCH, RR, U>, 64, lds_fwd32(smax));
     VD_OCC(1) VD_OCC(2) VD_OCC(3) VD_OCC(4) VD_OCC(5) VD_OCC(6) VD_OCC(7) VD_OCC(8)
     VD_OCC(9) VD_OCC(10) VD_OCC(11) VD_OCC(12) VD_OCC(13) VD_OCC(14) VD_OCC(15) VD_OCC(16)
 #undef VD_OCC
@@ -542,21 +545,24 @@ static hipError_t occ_fwd32(int r, int smax, int* n) {
   }
 }
 
-// blocks per CU of the FP32 kernel of class r (1..16), or of the FP64 kernel (r = 0)
-hipError_t occupancy(int anchored, int r, int smax, int* blocks_per_cu) {
+// blocks per CU of the FP32 kernel of class r (1..16) with main-loop unroll u,
+// or of the FP64 kernel (r = 0)
+hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu) {
   if (r == 0)
     return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<true>, 64, lds_fwd64(smax))
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_fwd64<false>, 64, lds_fwd64(smax));
-  return anchored ? occ_fwd32<true>(r, smax, blocks_per_cu) : occ_fwd32<false>(r, smax, blocks_per_cu);
+  if (u == 4)
+    return anchored ? occ_fwd32<true, 4>(r, smax, blocks_per_cu) : occ_fwd32<false, 4>(r, smax, blocks_per_cu);
+  return anchored ? occ_fwd32<true, 2>(r, smax, blocks_per_cu) : occ_fwd32<false, 2>(r, smax, blocks_per_cu);
 }
 
-template <bool ANCH>
+template <bool ANCH, int U>
 static void launch_fwd32_r(int r, int blocks, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
                            const QTab<float>* tab32, int smax, double* out) {
   switch (r) {
-#define VD_L(RR)                                                                                      \
-  case RR:                                                                                            \
-    hipLaunchKernelGGL((k_fwd32<ANCH, RR>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
+#define VD_L(RR)                                                                                         \
+  case RR:                                                                                               \
+    hipLaunchKernelGGL((k_fwd32<ANCH, RR, U>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
     break;
     VD_L(1) VD_L(2) VD_L(3) VD_L(4) VD_L(5) VD_L(6) VD_L(7) VD_L(8)
     VD_L(9) VD_L(10) VD_L(11) VD_L(12) VD_L(13) VD_L(14) VD_L(15) VD_L(16)
@@ -585,17 +591,24 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
-                           int blocks64, double* ws64, double* out, hipStream_t st, hipEvent_t* ev) {
+                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
+                           hipEvent_t* ev, hipEvent_t* ev_cls) {
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, force64, out, st);
   if (ev) (void)hipEventRecord(ev[1], st);
   if (!force64) {
     for (int r = MAX_R32; r >= 1; --r) {  // largest class first
-      if (anchored)
-        launch_fwd32_r<true>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
-      else
-        launch_fwd32_r<false>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+      if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
+      const bool u4 = unroll32[r] == 4;
+      if (anchored) {
+        if (u4) launch_fwd32_r<true, 4>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+        else launch_fwd32_r<true, 2>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+      } else {
+        if (u4) launch_fwd32_r<false, 4>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+        else launch_fwd32_r<false, 2>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+      }
     }
+    if (ev_cls) (void)hipEventRecord(ev_cls[0], st);
   }
   if (ev) (void)hipEventRecord(ev[2], st);
   if (anchored)
