@@ -189,6 +189,52 @@ def plain_baseline(batch, mode: int, pairs: int):
             "kind": "one kernel launch per pair, one thread per read row, FP32, inputs resident in HBM"}
 
 
+def accuracy_configs(mode: int):
+    """The single 64x8 region and the 1024-region precision-stress workload:
+    GCUPS, FP32 pairs rescued in FP64, and per-pair |log10 L - log10 L_FP64|
+    (max, p99).  The FP64 reference of the stress set is the device FP64 path,
+    itself checked against the CPU oracle on a 2-region sample; the single
+    region is checked against the oracle directly."""
+    import oracle
+    from vardepth_amd import engine, workloads
+    ln10 = math.log(10.0)
+    out = {}
+    b1 = workloads.single_region(seed=41001)
+    engine.forward(b1, mode=mode)  # warm
+    t0 = time.perf_counter()
+    m1, st1 = engine.forward(b1, mode=mode, precision="mixed", stats=True)
+    wall1 = time.perf_counter() - t0
+    ref1, _ = oracle.regions_cpu(b1, mode=mode, threads=4)
+    e1 = np.abs(m1 - ref1) / ln10
+    out["single_region"] = {"workload": "single region 64 reads x 8 haplotypes (seed 41001, g=10^-4.5, x=0.1)",
+                            "cells": b1.cells(), "device_ms": round(st1["device_ms"], 3),
+                            "gcups_device": round(b1.cells() / st1["device_ms"] / 1e6, 2),
+                            "wall_ms_with_transfers": round(wall1 * 1e3, 3),
+                            "max_log10_err_vs_oracle": float(e1.max()), "flagged": int(st1["flagged"])}
+    b3 = workloads.precision_stress(seed=41003, regions=1024)
+    dev = engine.DeviceBatch(b3)
+    dev.forward(mode=mode)
+    st3 = dev.forward(mode=mode, precision="mixed", stats=True)
+    m3 = dev.result()
+    st64 = dev.forward(mode=mode, precision="float64", stats=True)
+    f3 = dev.result()
+    ok = np.isfinite(f3)
+    e3 = np.abs(m3[ok] - f3[ok]) / ln10
+    sub = b3.subset([0, 1])
+    ref3, _ = oracle.regions_cpu(sub, mode=mode, threads=4)
+    n_sub = sub.npairs
+    e_or = np.abs(f3[:n_sub] - ref3) / ln10
+    out["precision_stress"] = {
+        "workload": "precision stress (1024 regions, 16 x 250 bp reads Phred 2-15, 2-4 x 1-2 kb haplotypes with "
+                    "an indel per 50 bp, seed 41003)",
+        "pairs": b3.npairs, "cells": b3.cells(), "fwd_ms": round(st3["device_ms"], 3),
+        "gcups": round(b3.cells() / st3["device_ms"] / 1e6, 2), "fp64_all_ms": round(st64["device_ms"], 3),
+        "flagged": int(st3["flagged"]), "flagged_frac": round(st3["flagged"] / b3.npairs, 4),
+        "max_log10_err_vs_fp64": float(e3.max()), "p99_log10_err_vs_fp64": float(np.quantile(e3, 0.99)),
+        "fp64_vs_cpu_oracle_max_log10": float(e_or.max()), "oracle_sample_pairs": int(n_sub)}
+    return out
+
+
 def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     """Forward and forward+adjoint GCUPS on the HMM-sampled workload (all pairs,
     unit weights: gradient of sum ln L wrt gap, ext and the 64-bin error table)."""
@@ -244,6 +290,7 @@ def main():
     ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -314,6 +361,11 @@ def main():
         print("plain one-pair-per-launch: %.3f GCUPS on %d pairs" % (plain["gcups"], plain["pairs"]),
               file=sys.stderr, flush=True)
 
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = accuracy_configs(args.mode)
+        print("accuracy: %s" % json.dumps(accuracy), file=sys.stderr, flush=True)
+
     fwd_bwd = None
     if args.grad_regions > 0:
         fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)
@@ -352,6 +404,7 @@ def main():
         "graph": graph,
         "plain": plain,
         "fwd_bwd": fwd_bwd,
+        "accuracy": accuracy,
     }
     print(json.dumps(line), flush=True)
 
