@@ -17,6 +17,12 @@ The JSON line carries:
 * plain      -- the one-pair-per-launch GPU forward on a sample of the workload
 * fwd_bwd    -- forward + adjoint (gradients wrt gap, ext, 64-bin error table)
                 on the 1024-region HMM-sampled workload (seed 41004, 2.05M pairs)
+* genotype   -- the chromosome-scale sample (seed 41005): forward then the
+                diploid genotype likelihood over every haplotype pair of every
+                400 bp region (normalized rows, certainty filter, molecule rows,
+                allele-balance + artifact mixture), end to end on the device,
+                at 30x and 2x depth; the CPU chain (C restatement of the
+                reference forward + genotype) timed on a bounded sample
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--regions N]
 """
@@ -272,6 +278,94 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "autotune": {"forward": tune_f, "adjoint": tune_g}}
 
 
+def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float, threads: int):
+    """Forward + diploid genotype likelihood on the chromosome-scale sample."""
+    import torch
+    import oracle
+    from vardepth_amd import engine, workloads
+    res = {}
+    for depth in depths:
+        t0 = time.perf_counter()
+        cs = workloads.chromosome_sample(mb=mb, depth=depth)
+        gen_s = time.perf_counter() - t0
+        b = cs.batch
+        cells = b.cells()
+        nh = np.diff(b.region_haps)
+        gl_off = engine.gl_offsets(nh)
+        dev = engine.DeviceBatch(b)
+        mq = torch.from_numpy(cs.mq).cuda()
+        dev.forward(mode=mode)
+        dev.genotype(mq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.forward(mode=mode)
+            gl = dev.genotype(mq)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        fst = dev.forward(mode=mode, stats=True)
+        gl, gst = dev.genotype(mq, stats=True)
+        g = gl.cpu().numpy()
+        ll = dev.result()
+        calls = np.array([int(np.argmax(g[gl_off[k]:gl_off[k + 1]])) for k in range(b.nregions)])
+        var = nh >= 2
+        # CPU chain on a time-bounded random sample of regions
+        rng = np.random.default_rng(5)
+        order = rng.permutation(b.nregions)
+        done, cpu_cells, cpu_t, maxdiff, stage_diff, agree, nvar = 0, 0, 0.0, 0.0, 0.0, 0, 0
+        while done < b.nregions and cpu_t < cpu_seconds:
+            ks = np.sort(order[done:done + 128])
+            done += len(ks)
+            sub = b.subset(ks)
+            snh = np.diff(sub.region_haps).astype(np.int32)
+            soff = engine.gl_offsets(snh)
+            smq = np.concatenate([cs.mq[b.region_reads[k]:b.region_reads[k + 1]] for k in ks])
+            t1 = time.perf_counter()
+            sll, _ = oracle.regions_cpu(sub, mode=mode, threads=threads)
+            sgl, _ = oracle.genotype_regions(sub.region_reads, snh, sub.out_off()[:-1], sll, smq, soff,
+                                             threads=threads)
+            cpu_t += time.perf_counter() - t1
+            cpu_cells += sub.cells()
+            # the device's genotype stage vs the CPU chain on the device's own likelihoods
+            oo = b.out_off()
+            dll = np.concatenate([ll[oo[k]:oo[k + 1]] for k in ks])
+            dgl, _ = oracle.genotype_regions(sub.region_reads, snh, sub.out_off()[:-1], dll, smq, soff,
+                                             threads=threads)
+            for i, k in enumerate(ks):
+                mine = g[gl_off[k]:gl_off[k + 1]]
+                maxdiff = max(maxdiff, float(np.max(np.abs(mine - sgl[soff[i]:soff[i + 1]]))))
+                stage_diff = max(stage_diff, float(np.max(np.abs(mine - dgl[soff[i]:soff[i + 1]]) /
+                                                          np.maximum(1, np.abs(dgl[soff[i]:soff[i + 1]])))))
+                if snh[i] >= 2:
+                    nvar += 1
+                    agree += int(np.argmax(sgl[soff[i]:soff[i + 1]]) == calls[k])
+        cpu_gcups = cpu_cells / cpu_t / 1e9 if cpu_t > 0 else None
+        res["%gx" % depth] = {
+            "workload": "chromosome sample %.3g Mb at %gx (seed 41005): %d regions, %d haplotypes (max %d), "
+                        "%d reads in regions" % (mb, depth, b.nregions, b.nhaps, int(nh.max()), b.nreads),
+            "regions": b.nregions, "pairs": b.npairs, "genotype_outputs": int(gl_off[-1]), "cells": cells,
+            "generate_s": round(gen_s, 1),
+            "end_to_end_ms": round(wall * 1e3, 3), "end_to_end_gcups": round(cells / wall / 1e9, 2),
+            "forward_ms": round(fst["device_ms"], 3), "molecule_ms": round(gst["molecule_ms"], 3),
+            "mixture_ms": round(gst["mixture_ms"], 3), "mixture_units": gst["units"],
+            "kept_rows": gst["kept_rows"], "invalid_regions": gst["invalid"],
+            "truth_concordance": round(float(np.mean(calls[var] == cs.true_pair[var])), 4) if var.any() else None,
+            "cpu": {"gcups": round(cpu_gcups, 3) if cpu_gcups else None, "threads": threads,
+                    "kind": "port", "sample_regions": int(done), "sample_cells": int(cpu_cells),
+                    "seconds": round(cpu_t, 2),
+                    "projected_wall_s": round(cells / (cpu_gcups * 1e9), 1) if cpu_gcups else None,
+                    "what": "oracle regions_cpu (C restatement of trie_many + pairhmm_pairs) + genotype chain"},
+            "speedup_vs_cpu": round(cells / wall / 1e9 / cpu_gcups, 1) if cpu_gcups else None,
+            "gl_max_abs_diff_vs_cpu_chain": maxdiff,
+            "genotype_stage_max_rel_diff": stage_diff,
+            "call_agreement_vs_cpu": round(agree / nvar, 4) if nvar else None,
+        }
+        print("genotype %gx: %s" % (depth, json.dumps(res["%gx" % depth])), file=sys.stderr, flush=True)
+        del dev, gl, mq
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +385,11 @@ def main():
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
+    ap.add_argument("--geno-mb", type=float, default=6.0,
+                    help="chromosome-sample size for the genotype leg, Mb (0 = skip; 60 = the full sample)")
+    ap.add_argument("--geno-depths", default="30,2", help="depth profiles of the genotype leg")
+    ap.add_argument("--geno-cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--only-genotype", action="store_true", help="run only the genotype leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -301,6 +400,12 @@ def main():
 
     import torch
     from vardepth_amd import engine
+
+    if args.only_genotype:
+        print(json.dumps({"genotype": genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")],
+                                                     args.mode, max(1, min(args.steps, 3)), args.geno_cpu_seconds,
+                                                     min(available_cpus(), 16))}), flush=True)
+        return
 
     if args.no_forward:
         print(json.dumps({"fwd_bwd": grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)}),
@@ -371,6 +476,11 @@ def main():
         fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)
         print("fwd+bwd: %s" % json.dumps(fwd_bwd), file=sys.stderr, flush=True)
 
+    genotype = None
+    if args.geno_mb > 0:
+        genotype = genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")], args.mode,
+                                  max(1, min(args.steps, 3)), args.geno_cpu_seconds, min(available_cpus(), 16))
+
     if args.copy_calib:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -405,6 +515,7 @@ def main():
         "plain": plain,
         "fwd_bwd": fwd_bwd,
         "accuracy": accuracy,
+        "genotype": genotype,
     }
     print(json.dumps(line), flush=True)
 

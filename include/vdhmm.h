@@ -179,6 +179,71 @@ int vd_tuning_set(const vd_tuning *t);
 void vd_class_timing(int on);
 int vd_class_ms(double *fwd_ms, double *grad_ms);
 
+/* ---- diploid genotype likelihood ---------------------------------------- */
+
+/* Replaces mixture_general (caller/native/caller/mixture_general.cpp:9, bound
+ * by mixture_cpu.py:18-21): same arguments and output.  lh is rows x n
+ * row-major (finite, >= 0); prior < 0: no mixture, 0: allele-balance mixture,
+ * > 0: balance and artifact mixture (requires prior < 1); triangular != 0:
+ * n(n+1)/2 outputs in the order b = 0..n-1, a = 0..b, else n*n a-major.  Host
+ * pointers.  Returns 0, or 2 with vd_last_error() set (the reference returns
+ * void; mixture_cpu.py:25 rejects the same inputs before the call). */
+int vd_mixture_general(int rows, int n, const double *lh, double prior, int triangular, double *out);
+
+/* Replaces mixture_two (mixture_two.cpp:9, bound by mixture_cpu.py:8-11):
+ * rows x 2 matrix, out[2a+b] for the ordered genotypes.  Host pointers. */
+int vd_mixture_two(int rows, const double *lh, double prior, double *out);
+
+/* Replaces molecule_rows.normalized_rows (molecule_rows.py:6-14): values =
+ * (1-e) exp(ll - rowmax) + e mean, e = 10^(-min(mq,60)/10), certainty =
+ * ptp(log(max(1e-300, values))) per row.  Host pointers, rows x n. */
+int vd_normalized_rows(int rows, int n, const double *ll, const unsigned char *mq, double *values,
+                       double *certainty);
+
+/* A batch of genotype regions in device memory.  Region k: rows
+ * row_off[k]..row_off[k+1] (global row indices into mq / the per-row
+ * outputs), haplotype columns col_off[k]..col_off[k+1], a row-major
+ * rows x columns block at mat_off[k] of the input matrix, and its pair
+ * outputs at gl_off[k].  mat_size / gl_size bound every block; a region
+ * whose block falls outside them is skipped and counted.  A vd_batch of the
+ * forward maps onto it as row_off = region_reads, col_off = region_haps,
+ * mat_off = out_off. */
+typedef struct vd_geno_batch {
+  const int64_t *row_off;
+  const int64_t *col_off;
+  const int64_t *mat_off;
+  const int64_t *gl_off;
+  int64_t nregions, nrows, mat_size, gl_size;
+} vd_geno_batch;
+
+typedef struct vd_geno_stats {
+  int64_t units;        /* mixture wave units (64 pairs each) */
+  int64_t kept_rows;    /* rows that passed the certainty filter */
+  int64_t invalid;      /* regions with a non-finite or negative matrix entry: outputs NaN */
+  int64_t skipped;      /* regions outside the batch bounds: outputs untouched */
+  double molecule_ms, mixture_ms, total_ms;
+} vd_geno_stats;
+
+/* The diploid likelihood of every region of a batch of matrices (device
+ * pointers; the batched form of vd_mixture_general with triangular order or
+ * square).  keep (optional, per row): rows with keep == 0 are left out.
+ * stream: a hipStream_t or NULL for the library's stream.  stats (optional)
+ * synchronises and fills the counters and times. */
+int vd_mixture_regions_device(const vd_geno_batch *batch, const double *lh, const signed char *keep,
+                              double prior, int triangular, double *gl, void *stream,
+                              vd_geno_stats *stats);
+
+/* The single-end read-evidence chain of whole_read.py:30-49 on the device:
+ * for each region of a log-likelihood batch ll (the forward's output layout),
+ * normalized rows with per-row mapping quality mq, rows with certainty below
+ * min_certainty dropped (whole_read.py:37 uses ln 3; NaN is kept), molecule
+ * rows exp(log(max(1e-300, v)) - max), then the triangular diploid likelihood
+ * with the given prior (whole_read.py:49 uses 1e-3) into gl.  values
+ * (rows x columns, like ll) and certainty (per row) are optional outputs. */
+int vd_genotype_device(const vd_geno_batch *batch, const double *ll, const unsigned char *mq,
+                       double min_certainty, double prior, double *gl, double *values, double *certainty,
+                       void *stream, vd_geno_stats *stats);
+
 /* ---- baseline ----------------------------------------------------------- */
 
 /* The plain GPU forward: the first max_pairs pairs of the batch (region

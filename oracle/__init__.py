@@ -11,6 +11,10 @@ for the product path.
 * :func:`ref` -- the reference's own C++ sources compiled by
   ``oracle/Makefile`` into ``oracle/_ref`` (absent unless built).
 * :mod:`oracle.brute` -- exact rational sum over alignments for tiny pairs.
+* :func:`normalized_rows`, :func:`mixture_general`, :func:`mixture_two`,
+  :func:`genotype_regions` -- the diploid genotype likelihood chain
+  (``genotype.c``: molecule_rows.py, mixture_general.cpp, mixture_two.cpp,
+  triangular.py, whole_read.py of caller/native/caller).
 """
 from __future__ import annotations
 
@@ -60,6 +64,15 @@ def lib():
                                   ctypes.c_char_p, _u8p, ctypes.c_char_p, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int, _dp, _u64p]
     L.vdo_regions_cpu.restype = ctypes.c_int
+    L.vdo_normalized_rows.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _u8p, _dp, _dp]
+    L.vdo_normalized_rows.restype = None
+    L.vdo_mixture_general.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _dp]
+    L.vdo_mixture_general.restype = None
+    L.vdo_mixture_two.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp]
+    L.vdo_mixture_two.restype = None
+    L.vdo_genotype_regions.argtypes = [ctypes.c_int64, _i64p, ctypes.POINTER(ctypes.c_int32), _i64p, _dp,
+                                       _u8p, ctypes.c_double, ctypes.c_double, _i64p, ctypes.c_int, _dp]
+    L.vdo_genotype_regions.restype = ctypes.c_int64
     return L
 
 
@@ -143,6 +156,75 @@ def regions_cpu(batch, mode=0, err=None, threads=None):
 
 
 # ---------------------------------------------------------------------------
+# diploid genotype likelihood (genotype.c)
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def normalized_rows(ll, mq):
+    """molecule_rows.py:6-14 -> (values, certainty)."""
+    x = _f64(ll)
+    rows, n = x.shape
+    q = np.ascontiguousarray(mq, np.uint8)
+    values = np.empty_like(x)
+    cert = np.empty(rows, np.float64)
+    if rows and n:
+        lib().vdo_normalized_rows(rows, n, x.ctypes.data_as(_dp), q.ctypes.data_as(_u8p),
+                                  values.ctypes.data_as(_dp), cert.ctypes.data_as(_dp))
+    return values, cert
+
+
+def mixture_general(lh, prior=None, triangular=True):
+    """mixture_general.cpp:9-40 (prior None -> -1, as mixture_cpu.py:27)."""
+    x = _f64(lh)
+    rows, n = x.shape
+    out = np.empty(n * (n + 1) // 2 if triangular else n * n)
+    lib().vdo_mixture_general(rows, n, x.ctypes.data_as(_dp), -1.0 if prior is None else float(prior),
+                              int(bool(triangular)), out.ctypes.data_as(_dp))
+    return out
+
+
+def mixture_two(lh, prior):
+    """mixture_two.cpp:9-29 -> 2 x 2 ordered genotypes."""
+    x = _f64(lh)
+    out = np.empty(4)
+    lib().vdo_mixture_two(len(x), x.ctypes.data_as(_dp), float(prior), out.ctypes.data_as(_dp))
+    return out.reshape(2, 2)
+
+
+def diploid_likelihood(lh, artifact_prior=None):
+    """triangular.py:6-11."""
+    x = _f64(lh)
+    if x.shape[1] == 2 and artifact_prior is not None:
+        return mixture_two(x, artifact_prior).ravel()[[0, 1, 3]]
+    return mixture_general(x, artifact_prior, True)
+
+
+def genotype_regions(row_off, ncol, mat_off, ll, mq, gl_off, min_certainty=float(np.log(3)),
+                     prior=0.001, threads=None):
+    """whole_read.py:30-49 single-end chain over a region batch -> (gl, invalid regions)."""
+    row_off = np.ascontiguousarray(row_off, np.int64)
+    ncol = np.ascontiguousarray(ncol, np.int32)
+    mat_off = np.ascontiguousarray(mat_off, np.int64)
+    gl_off = np.ascontiguousarray(gl_off, np.int64)
+    ll = _f64(ll)
+    mq = np.ascontiguousarray(mq, np.uint8)
+    nreg = len(ncol)
+    total = int(gl_off[-1]) if len(gl_off) > nreg else int(
+        gl_off[-1] + ncol[-1] * (ncol[-1] + 1) // 2) if nreg else 0
+    gl = np.full(max(total, 1), np.nan)
+    if threads is None:
+        threads = os.cpu_count() or 1
+    bad = lib().vdo_genotype_regions(nreg, row_off.ctypes.data_as(_i64p),
+                                     ncol.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                     mat_off.ctypes.data_as(_i64p), ll.ctypes.data_as(_dp),
+                                     mq.ctypes.data_as(_u8p), float(min_certainty), float(prior),
+                                     gl_off.ctypes.data_as(_i64p), int(threads), gl.ctypes.data_as(_dp))
+    return gl[:total], int(bad)
+
+
+# ---------------------------------------------------------------------------
 # the reference's own sources compiled into oracle/_ref (never shipped)
 
 class Ref:
@@ -176,6 +258,16 @@ class Ref:
         self.anchored = ctypes.CDLL(str(root / "libanchored.so"))
         self.anchored.anchored_pairhmm.argtypes = args7
         self.anchored.anchored_pairhmm.restype = ctypes.c_double
+        self.mix_two = self.mix_general = None
+        if (root / "libmixture_two.so").exists():   # mixture_cpu.py:10,20
+            self.mix_two = ctypes.CDLL(str(root / "libmixture_two.so"))
+            self.mix_two.mixture_two.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp]
+            self.mix_two.mixture_two.restype = None
+        if (root / "libmixture_general.so").exists():
+            self.mix_general = ctypes.CDLL(str(root / "libmixture_general.so"))
+            self.mix_general.mixture_general.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                                         ctypes.c_int, _dp]
+            self.mix_general.mixture_general.restype = None
 
     # convenience wrappers ------------------------------------------------
     def pairhmm_pairs(self, jobs, mode, lanes=0):
@@ -206,6 +298,20 @@ class Ref:
         if code:
             raise RuntimeError("Vector prefix PairHMM failed: %d" % code)
         return out, int(cells.value)
+
+    def mixture_general(self, lh, prior=None, triangular=True):
+        x = _f64(lh)
+        rows, n = x.shape
+        out = np.empty(n * (n + 1) // 2 if triangular else n * n)
+        self.mix_general.mixture_general(rows, n, x.ctypes.data_as(_dp), -1.0 if prior is None else float(prior),
+                                         int(bool(triangular)), out.ctypes.data_as(_dp))
+        return out
+
+    def mixture_two(self, lh, prior):
+        x = _f64(lh)
+        out = np.empty(4)
+        self.mix_two.mixture_two(len(x), x.ctypes.data_as(_dp), float(prior), out.ctypes.data_as(_dp))
+        return out.reshape(2, 2)
 
     def scalar(self, fn, jobs, gap=0.001, ext=0.1):
         f = {"wide": self.wide.read_haplotype, "core": self.core.read_haplotype,

@@ -29,7 +29,8 @@ SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch
            "read_haplotype_batch_mode", "wide_capability", "anchored_pairhmm",
            "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
            "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_tuning_get",
-           "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_last_error", "vd_device")
+           "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_mixture_general", "vd_mixture_two",
+           "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_last_error", "vd_device")
 
 
 class VdParams(ctypes.Structure):
@@ -60,6 +61,21 @@ class VdTuning(ctypes.Structure):
 
     def as_dict(self):
         return {name: list(getattr(self, name)) for name, _ in self._fields_}
+
+
+class VdGenoBatch(ctypes.Structure):
+    _fields_ = [("row_off", ctypes.c_void_p), ("col_off", ctypes.c_void_p), ("mat_off", ctypes.c_void_p),
+                ("gl_off", ctypes.c_void_p), ("nregions", ctypes.c_int64), ("nrows", ctypes.c_int64),
+                ("mat_size", ctypes.c_int64), ("gl_size", ctypes.c_int64)]
+
+
+class VdGenoStats(ctypes.Structure):
+    _fields_ = [("units", ctypes.c_int64), ("kept_rows", ctypes.c_int64), ("invalid", ctypes.c_int64),
+                ("skipped", ctypes.c_int64), ("molecule_ms", ctypes.c_double), ("mixture_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class VdhmmError(RuntimeError):
@@ -120,6 +136,20 @@ def lib() -> ctypes.CDLL:
     L.vd_class_timing.restype = None
     L.vd_class_ms.argtypes = [_dp, _dp]
     L.vd_class_ms.restype = ctypes.c_int
+    L.vd_mixture_general.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _dp]
+    L.vd_mixture_general.restype = ctypes.c_int
+    L.vd_mixture_two.argtypes = [ctypes.c_int, _dp, ctypes.c_double, _dp]
+    L.vd_mixture_two.restype = ctypes.c_int
+    L.vd_normalized_rows.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _u8p, _dp, _dp]
+    L.vd_normalized_rows.restype = ctypes.c_int
+    L.vd_mixture_regions_device.argtypes = [ctypes.POINTER(VdGenoBatch), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(VdGenoStats)]
+    L.vd_mixture_regions_device.restype = ctypes.c_int
+    L.vd_genotype_device.argtypes = [ctypes.POINTER(VdGenoBatch), ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(VdGenoStats)]
+    L.vd_genotype_device.restype = ctypes.c_int
     L.vd_last_error.argtypes = []
     L.vd_last_error.restype = ctypes.c_char_p
     L.vd_device.argtypes = []

@@ -564,6 +564,211 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
   return 0;
 }
 
+// ---- diploid genotype likelihood ------------------------------------------
+
+struct GenoBufs {
+  DevBuf mol, keep, units, map, ctr, mq_err;
+  DevBuf in_off, in_lh, in_gl, in_mq, in_vals, in_cert;  // host entry points
+  bool mq_err_ready = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+GenoBufs gg;
+
+int geno_init(hipStream_t st) {
+  if (!gg.ev[0])
+    for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&gg.ev[i]));
+  if (!gg.mq_err_ready) {
+    // molecule_rows.py:4: _MQ_ERROR[mq] = 10**(-min(mq,60)/10)
+    static double t[256];
+    for (int q = 0; q < 256; ++q) t[q] = std::pow(10., -std::min(q, 60) / 10.);
+    HIPCHK(gg.mq_err.ensure(sizeof(t)));
+    HIPCHK(hipMemcpyAsync(gg.mq_err.p, t, sizeof(t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    gg.mq_err_ready = true;
+  }
+  HIPCHK(gg.ctr.ensure(sizeof(unsigned long long) * vd::GCTR_COUNT));
+  return 0;
+}
+
+int check_geno_batch(const vd_geno_batch* b) {
+  if (!b) return fail("vdhmm: null genotype batch");
+  if (b->nregions < 0 || b->nrows < 0 || b->mat_size < 0 || b->gl_size < 0)
+    return fail("vdhmm: negative genotype batch size");
+  if (b->nregions > 0 && (!b->row_off || !b->col_off || !b->mat_off || !b->gl_off))
+    return fail("vdhmm: null genotype batch offsets");
+  if (b->nregions > INT32_MAX) return fail("vdhmm: more than 2^31-1 genotype regions");
+  return 0;
+}
+
+int check_prior(double prior, bool allow_none) {
+  if (std::isnan(prior) || prior >= 1 || (!allow_none && prior < 0))
+    return fail("vdhmm: invalid mixture prior (0 <= prior < 1, or negative for none)");
+  return 0;
+}
+
+vd::GenoDev geno_dev(const vd_geno_batch* b) {
+  vd::GenoDev d;
+  d.row_off = b->row_off;
+  d.col_off = b->col_off;
+  d.mat_off = b->mat_off;
+  d.gl_off = b->gl_off;
+  d.nregions = b->nregions;
+  d.nrows = b->nrows;
+  d.mat_size = b->mat_size;
+  d.gl_size = b->gl_size;
+  d.unit_cap = b->nregions + b->gl_size / 64 + 1;  // sum of ceil(pairs/64) when blocks do not overlap
+  return d;
+}
+
+int geno_zero_ctr(hipStream_t st) {
+  // a fill kernel rather than a memset node, so a captured graph replays it
+  HIPCHK(vd::launch_zero_u64(gg.ctr.as<unsigned long long>(), vd::GCTR_COUNT, st));
+  return 0;
+}
+
+int geno_stats(hipStream_t st, vd_geno_stats* s, bool molecule) {
+  if (!s) return 0;
+  HIPCHK(hipStreamSynchronize(st));
+  unsigned long long c[vd::GCTR_COUNT];
+  HIPCHK(hipMemcpy(c, gg.ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+  s->units = (int64_t)c[vd::GCTR_UNITS];
+  s->invalid = (int64_t)c[vd::GCTR_INVALID];
+  s->kept_rows = (int64_t)c[vd::GCTR_KEPT];
+  s->skipped = (int64_t)c[vd::GCTR_LAYOUT];
+  float ms = 0;
+  if (molecule) {
+    HIPCHK(hipEventElapsedTime(&ms, gg.ev[0], gg.ev[1]));
+    s->molecule_ms = ms;
+  }
+  HIPCHK(hipEventElapsedTime(&ms, gg.ev[1], gg.ev[2]));
+  s->mixture_ms = ms;
+  return 0;
+}
+
+int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t* keep, double prior, int tri,
+                          double* gl, hipStream_t st, vd_geno_stats* s) {
+  if (int rc = check_geno_batch(b)) return rc;
+  if (int rc = check_prior(prior, true)) return rc;
+  if (b->nregions > 0 && ((!lh && b->mat_size > 0) || (!gl && b->gl_size > 0)))
+    return fail("vdhmm: null genotype matrix or output");
+  if (int rc = engine_init()) return rc;
+  if (!st) st = g.own_stream;
+  if (int rc = geno_init(st)) return rc;
+  const vd::GenoDev d = geno_dev(b);
+  HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
+  HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
+  if (int rc = geno_zero_ctr(st)) return rc;
+  if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
+  HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, gg.units.as<int64_t>(), gg.map.as<int32_t>(), gl,
+                            gg.ctr.as<unsigned long long>(), g.cus, st));
+  if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
+  return geno_stats(st, s, false);
+}
+
+int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8_t* mq, double min_cert,
+                           double prior, double* gl, double* values, double* cert, hipStream_t st,
+                           vd_geno_stats* s) {
+  if (int rc = check_geno_batch(b)) return rc;
+  if (int rc = check_prior(prior, true)) return rc;
+  if (std::isnan(min_cert)) return fail("vdhmm: min_certainty is NaN");
+  if (b->nregions > 0 && ((!ll && b->mat_size > 0) || (!mq && b->nrows > 0) || (!gl && b->gl_size > 0)))
+    return fail("vdhmm: null genotype input or output");
+  if (int rc = engine_init()) return rc;
+  if (!st) st = g.own_stream;
+  if (int rc = geno_init(st)) return rc;
+  const vd::GenoDev d = geno_dev(b);
+  HIPCHK(gg.mol.ensure(sizeof(double) * std::max<int64_t>(b->mat_size, 1)));
+  HIPCHK(gg.keep.ensure(std::max<int64_t>(b->nrows, 1)));
+  HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
+  HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
+  if (int rc = geno_zero_ctr(st)) return rc;
+  if (s) HIPCHK(hipEventRecord(gg.ev[0], st));
+  HIPCHK(vd::launch_molecule(d, ll, mq, gg.mq_err.as<double>(), min_cert, values, cert, gg.mol.as<double>(),
+                             gg.keep.as<int8_t>(), gg.ctr.as<unsigned long long>(), g.cus, st));
+  if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
+  HIPCHK(vd::launch_mixture(d, gg.mol.as<double>(), gg.keep.as<int8_t>(), prior, 1, gg.units.as<int64_t>(),
+                            gg.map.as<int32_t>(), gl, gg.ctr.as<unsigned long long>(), g.cus, st));
+  if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
+  return geno_stats(st, s, true);
+}
+
+// one region of rows x n staged from host memory
+int stage_one_region(int rows, int n, int64_t pairs, vd_geno_batch* b, hipStream_t st) {
+  const int64_t off[8] = {0, rows, 0, n, 0, 0, 0, 0};  // row_off[2], col_off[2], mat_off, gl_off
+  HIPCHK(gg.in_off.ensure(sizeof(off)));
+  HIPCHK(hipMemcpyAsync(gg.in_off.p, off, sizeof(off), hipMemcpyHostToDevice, st));
+  const int64_t* o = gg.in_off.as<int64_t>();
+  b->row_off = o;
+  b->col_off = o + 2;
+  b->mat_off = o + 4;
+  b->gl_off = o + 5;
+  b->nregions = 1;
+  b->nrows = rows;
+  b->mat_size = (int64_t)rows * n;
+  b->gl_size = pairs;
+  return 0;
+}
+
+int check_lh(int rows, int n, const double* lh) {  // mixture_cpu.py:14,25
+  if (n < 1 || rows < 0) return fail("vdhmm: invalid mixture likelihood shape");
+  if (rows > 0 && !lh) return fail("vdhmm: null mixture likelihood");
+  const int64_t cnt = (int64_t)rows * n;
+  for (int64_t i = 0; i < cnt; ++i)
+    if (!std::isfinite(lh[i]) || lh[i] < 0) return fail("vdhmm: invalid mixture likelihood (non-finite or negative)");
+  return 0;
+}
+
+int mixture_host_locked(int rows, int n, const double* lh, double prior, int tri, double* out) {
+  if (int rc = check_lh(rows, n, lh)) return rc;
+  if (int rc = check_prior(prior, true)) return rc;
+  if (!out) return fail("vdhmm: null output");
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  if (int rc = geno_init(st)) return rc;
+  const int64_t pairs = tri ? (int64_t)n * (n + 1) / 2 : (int64_t)n * n;
+  const int64_t cnt = (int64_t)rows * n;
+  vd_geno_batch b;
+  if (int rc = stage_one_region(rows, n, pairs, &b, st)) return rc;
+  HIPCHK(gg.in_lh.ensure(sizeof(double) * std::max<int64_t>(cnt, 1)));
+  HIPCHK(gg.in_gl.ensure(sizeof(double) * pairs));
+  if (cnt) HIPCHK(hipMemcpyAsync(gg.in_lh.p, lh, sizeof(double) * cnt, hipMemcpyHostToDevice, st));
+  if (int rc = mixture_device_locked(&b, gg.in_lh.as<double>(), nullptr, prior, tri, gg.in_gl.as<double>(), st,
+                                     nullptr))
+    return rc;
+  HIPCHK(hipMemcpyAsync(out, gg.in_gl.p, sizeof(double) * pairs, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int normalized_host_locked(int rows, int n, const double* ll, const uint8_t* mq, double* values, double* cert) {
+  if (rows < 0 || n < 0) return fail("vdhmm: negative shape");
+  if (rows == 0) return 0;
+  if (n < 1) return fail("vdhmm: normalized rows need at least one column");
+  if (!ll || !mq || !values || !cert) return fail("vdhmm: null argument");
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  if (int rc = geno_init(st)) return rc;
+  const int64_t cnt = (int64_t)rows * n;
+  vd_geno_batch b;
+  if (int rc = stage_one_region(rows, n, 0, &b, st)) return rc;
+  HIPCHK(gg.in_lh.ensure(sizeof(double) * cnt));
+  HIPCHK(gg.in_mq.ensure(rows));
+  HIPCHK(gg.in_vals.ensure(sizeof(double) * cnt));
+  HIPCHK(gg.in_cert.ensure(sizeof(double) * rows));
+  HIPCHK(gg.mol.ensure(sizeof(double) * cnt));
+  HIPCHK(gg.keep.ensure(rows));
+  HIPCHK(hipMemcpyAsync(gg.in_lh.p, ll, sizeof(double) * cnt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(gg.in_mq.p, mq, rows, hipMemcpyHostToDevice, st));
+  const vd::GenoDev d = geno_dev(&b);
+  HIPCHK(vd::launch_molecule(d, gg.in_lh.as<double>(), gg.in_mq.as<uint8_t>(), gg.mq_err.as<double>(), -INFINITY,
+                             gg.in_vals.as<double>(), gg.in_cert.as<double>(), gg.mol.as<double>(),
+                             gg.keep.as<int8_t>(), nullptr, g.cus, st));
+  HIPCHK(hipMemcpyAsync(values, gg.in_vals.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(cert, gg.in_cert.p, sizeof(double) * rows, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 // ---- helpers for the drop-ins ---------------------------------------------
 
 struct HostBatch {
@@ -735,6 +940,48 @@ int vd_forward_plain(const vd_batch* batch, const vd_params* params, int mode, i
   if (!batch || !params || !out) return fail("vdhmm: null argument");
   std::lock_guard<std::mutex> lk(g_mu);
   return plain_host_locked(batch, params, mode, max_pairs, out, gpu_ms);
+}
+
+int vd_mixture_general(int rows, int n, const double* lh, double prior, int triangular, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return mixture_host_locked(rows, n, lh, prior, triangular ? 1 : 0, out);
+}
+
+int vd_mixture_two(int rows, const double* lh, double prior, double* out) {
+  if (int rc = check_prior(prior, false)) return rc;  // mixture_cpu.py:14: 0 <= prior < 1
+  std::lock_guard<std::mutex> lk(g_mu);
+  return mixture_host_locked(rows, 2, lh, prior, 0, out);
+}
+
+int vd_normalized_rows(int rows, int n, const double* ll, const unsigned char* mq, double* values,
+                       double* certainty) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return normalized_host_locked(rows, n, ll, mq, values, certainty);
+}
+
+int vd_mixture_regions_device(const vd_geno_batch* batch, const double* lh, const signed char* keep, double prior,
+                              int triangular, double* gl, void* stream, vd_geno_stats* stats) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto t0 = std::chrono::steady_clock::now();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc = mixture_device_locked(batch, lh, (const int8_t*)keep, prior, triangular ? 1 : 0, gl, (hipStream_t)stream,
+                                 stats);
+  if (stats && !rc)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int vd_genotype_device(const vd_geno_batch* batch, const double* ll, const unsigned char* mq, double min_certainty,
+                       double prior, double* gl, double* values, double* certainty, void* stream,
+                       vd_geno_stats* stats) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto t0 = std::chrono::steady_clock::now();
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  int rc = genotype_device_locked(batch, ll, mq, min_certainty, prior, gl, values, certainty, (hipStream_t)stream,
+                                  stats);
+  if (stats && !rc)
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
 }
 
 int vd_tuning_get(vd_tuning* t) {

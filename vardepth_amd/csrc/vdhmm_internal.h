@@ -106,6 +106,29 @@ enum {
   CTR_COUNT = 40
 };
 
+// Device view of a genotype batch: region k owns rows row_off[k]..row_off[k+1]
+// (global row index: mapping quality and keep flag), haplotypes
+// col_off[k]..col_off[k+1], the row-major rows x haplotypes block at
+// mat_off[k] of the likelihood / molecule matrices, and its pair outputs at
+// gl_off[k].
+struct GenoDev {
+  const int64_t* row_off;
+  const int64_t* col_off;
+  const int64_t* mat_off;
+  const int64_t* gl_off;
+  int64_t nregions, nrows;
+  int64_t mat_size, gl_size;  // entries of the matrices / of the pair outputs (bounds of every block)
+  int64_t unit_cap;           // capacity of the wave-unit map
+};
+
+enum {
+  GCTR_UNITS = 0,    // mixture wave units (written by the scan)
+  GCTR_INVALID = 1,  // regions whose matrix held a non-finite or negative entry
+  GCTR_KEPT = 2,     // rows that passed the certainty filter
+  GCTR_LAYOUT = 3,   // regions skipped: block outside the matrix / output bounds, or unit map full
+  GCTR_COUNT = 4
+};
+
 struct LaunchCfg {
   int blocks32;   // persistent FP32 grid (one wave per block)
   int blocks64;   // persistent FP64 grid
@@ -135,4 +158,11 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
                        const int* blocks32, const int* variant32, const int* blocks64, int max_blocks, float* ck32,
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
                        double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls);
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq, const double* mq_err,
+                           double min_cert, double* values, double* cert, double* mol, int8_t* keep,
+                           unsigned long long* ctr, int cus, hipStream_t st);
+hipError_t launch_mixture(const GenoDev& g, const double* mat, const int8_t* keep, double prior, int triangular,
+                          int64_t* unit_off, int32_t* unit_region, double* gl, unsigned long long* ctr, int cus,
+                          hipStream_t st);
 }  // namespace vd

@@ -102,6 +102,17 @@ def forward_plain(batch: RegionBatch, mode=0, err=None, max_pairs=None):
     return out[:batch.npairs], ms.value
 
 
+GENO_MIN_CERTAINTY = float(np.log(3))   # whole_read.py:37
+GENO_PRIOR = 1e-3                        # whole_read.py:49
+
+
+def gl_offsets(ncols, triangular=True) -> np.ndarray:
+    """Pair-output offsets of regions with ``ncols`` haplotypes (length nregions+1)."""
+    n = np.asarray(ncols, np.int64)
+    pairs = n * (n + 1) // 2 if triangular else n * n
+    return np.concatenate([[0], np.cumsum(pairs)]).astype(np.int64)
+
+
 class DeviceBatch:
     """A RegionBatch copied once into HBM; forward() runs the device pipeline only."""
 
@@ -170,6 +181,48 @@ class DeviceBatch:
                                          ctypes.byref(st) if st is not None else None)
         _lib.check(code, "vd_grad_device")
         return (g, st.as_dict()) if st is not None else g
+
+    def genotype(self, mq, min_certainty=GENO_MIN_CERTAINTY, prior=GENO_PRIOR, stream=None, stats=False,
+                 values=False):
+        """Enqueue the single-end genotype chain of whole_read.py:30-49 on the
+        likelihoods in self.out (run forward() first): normalized rows with the
+        per-read mapping qualities ``mq`` (uint8 device tensor, one per read),
+        the certainty filter, molecule rows and the triangular diploid
+        likelihood.  Returns the float64 device tensor of every region's pairs
+        (region k at gl_off[k], b = 0..n-1, a = 0..b), plus (values, certainty)
+        device tensors when ``values`` and the stats dict when ``stats``."""
+        torch = self.torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        if mq.numel() != self.batch.nreads:
+            raise ValueError("mq must hold one mapping quality per read")
+        mq = mq.to(device=self.out.device, dtype=torch.uint8).contiguous()
+        if not hasattr(self, "_gl_off"):
+            off = gl_offsets(np.diff(self.batch.region_haps))
+            self.gl_size = int(off[-1])
+            self._gl_off = torch.from_numpy(off).to(self.out.device)
+            self.gb = _lib.VdGenoBatch(self.tensors[5].data_ptr(), self.tensors[6].data_ptr(),
+                                       self.tensors[7].data_ptr(), self._gl_off.data_ptr(), self.batch.nregions,
+                                       self.batch.nreads, self.npairs, self.gl_size)
+        gl = torch.empty(max(self.gl_size, 1), dtype=torch.float64, device=self.out.device)
+        vals = cert = None
+        if values:
+            vals = torch.empty(max(self.npairs, 1), dtype=torch.float64, device=self.out.device)
+            cert = torch.empty(max(self.batch.nreads, 1), dtype=torch.float64, device=self.out.device)
+        st = _lib.VdGenoStats() if stats else None
+        code = _lib.lib().vd_genotype_device(ctypes.byref(self.gb), ctypes.c_void_p(self.out.data_ptr()),
+                                             ctypes.c_void_p(mq.data_ptr()), float(min_certainty), float(prior),
+                                             ctypes.c_void_p(gl.data_ptr()),
+                                             ctypes.c_void_p(vals.data_ptr()) if values else None,
+                                             ctypes.c_void_p(cert.data_ptr()) if values else None,
+                                             ctypes.c_void_p(s.cuda_stream),
+                                             ctypes.byref(st) if st is not None else None)
+        _lib.check(code, "vd_genotype_device")
+        res = [gl[:self.gl_size]]
+        if values:
+            res += [vals[:self.npairs], cert[:self.batch.nreads]]
+        if stats:
+            res.append(st.as_dict())
+        return res[0] if len(res) == 1 else tuple(res)
 
     def result(self) -> np.ndarray:
         return self.out[:self.npairs].cpu().numpy()

@@ -19,6 +19,10 @@ The generators restate the benchmark workloads as concrete seeded inputs
   indel-dense haplotypes at Phred 2-15.
 * :func:`hmm_sampled_regions` -- reads sampled from the PairHMM itself (so
   gap parameters are identifiable), with the true haplotype of every read.
+* :func:`chromosome_sample` -- a seeded random chromosome with heterozygous
+  SNVs and indels on two haplotypes, 150 bp reads at a chosen depth, and
+  400 bp active regions whose candidate haplotypes are every combination of
+  the variant and candidate sites inside them (1..64 haplotypes).
 """
 from __future__ import annotations
 
@@ -96,13 +100,9 @@ class RegionBatch:
 
     def cells(self) -> int:
         """Sum over pairs of n*m (the reference's dp_cells, accelerator.py:155)."""
-        rl, hl = self.read_lengths(), self.hap_lengths()
-        total = 0
-        for k in range(self.nregions):
-            a = int(rl[self.region_reads[k]:self.region_reads[k + 1]].sum())
-            b = int(hl[self.region_haps[k]:self.region_haps[k + 1]].sum())
-            total += a * b
-        return total
+        a = self.read_off[self.region_reads[1:]] - self.read_off[self.region_reads[:-1]]
+        b = self.hap_off[self.region_haps[1:]] - self.hap_off[self.region_haps[:-1]]
+        return int(np.sum(a * b))
 
     def region(self, k):
         """(reads, quals, haps) of region k as Python str/bytes lists."""
@@ -401,3 +401,195 @@ def true_pairs(batch: RegionBatch) -> RegionBatch:
             if len(sel):
                 regions.append(([reads[i] for i in sel], [quals[i] for i in sel], [haps[h]]))
     return from_regions(regions, batch.gap, batch.ext)
+
+
+# ---------------------------------------------------------------------------
+# chromosome-scale diploid sample
+
+@dataclasses.dataclass
+class ChromosomeSample:
+    """A region batch with what the genotype chain needs besides it."""
+
+    batch: RegionBatch
+    mq: np.ndarray           # uint8 mapping quality of every batch read
+    true_pair: np.ndarray    # per region: triangular index of the true genotype (b = 0..n-1, a = 0..b)
+    region_start: np.ndarray  # per region: reference start of the window
+    info: dict
+
+
+def _apply_sites(ref: np.ndarray, lo: int, hi: int, pos, rlen, alt, pieces_out=None):
+    """ref[lo:hi] with the (sorted, non-overlapping) sites applied: site i
+    replaces ref[pos[i] : pos[i] + rlen[i]] by the bytes alt[i]."""
+    out, cur = [], lo
+    for p, r, a in zip(pos, rlen, alt):
+        out.append(ref[cur:p])
+        if pieces_out is not None:
+            pieces_out.append((cur, p - cur, True))
+        out.append(a)
+        if pieces_out is not None:
+            pieces_out.append((p, len(a), False))
+        cur = p + r
+    out.append(ref[cur:hi])
+    if pieces_out is not None:
+        pieces_out.append((cur, hi - cur, True))
+    return np.concatenate(out) if len(out) > 1 else ref[lo:hi].copy()
+
+
+def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, read_len: int = 150,
+                      window: int = 400, padding: int = 200, snv_rate: float = 1 / 1500,
+                      indel_rate: float = 1 / 10000, max_indel: int = 10, false_sites: float = 1.0,
+                      max_haps: int = 64, gap: float = GAP_Q45, ext: float = EXT_Q10) -> ChromosomeSample:
+    """Seeded chromosome-scale diploid sample.
+
+    * reference: ``mb`` million uniform random bases;
+    * two haplotypes: heterozygous SNVs at ``snv_rate`` and indels (1..max_indel
+      bp, insertion or deletion) at ``indel_rate`` per base, each on one
+      haplotype, sites at least 12 bp apart;
+    * reads: ``read_len`` bp at ``depth`` (half from each haplotype) at uniform
+      starts, Phred 85% U{30..40} / 15% U{10..29} with substitutions drawn
+      from them, mapping quality 60 (95%) or U{20..59};
+    * regions: consecutive ``window`` bp windows; candidate sites = the true
+      sites in the window plus Poisson(``false_sites``) candidate SNVs; the
+      haplotypes are ref[start - padding : end + padding] with every subset of
+      the sites applied (subset mask order, so haplotype 0 is the reference),
+      trimmed to at most ``max_haps`` by dropping candidate sites first;
+    * each region holds the reads overlapping its window, in start order
+      (whole_read.py:16-27 fetches the overlapping reads; with reads shorter
+      than the padding every one lies inside the padded window).
+    """
+    rng = np.random.default_rng(seed)
+    L = int(round(mb * 1e6))
+    ref = BASES[rng.integers(0, 4, L)]
+    lo_ok, hi_ok = padding + 16, L - padding - 16
+    # --- heterozygous sites
+    nsnv = rng.poisson(L * snv_rate)
+    nind = rng.poisson(L * indel_rate)
+    pos = np.concatenate([rng.integers(lo_ok, hi_ok, nsnv), rng.integers(lo_ok, hi_ok, nind)])
+    kind = np.concatenate([np.zeros(nsnv, np.int8), np.ones(nind, np.int8)])
+    order = np.argsort(pos, kind="stable")
+    pos, kind = pos[order], kind[order]
+    keep = np.ones(len(pos), bool)
+    keep[1:] = np.diff(pos) >= 12
+    pos, kind = pos[keep], kind[keep]
+    nsite = len(pos)
+    phase = rng.integers(0, 2, nsite).astype(np.int8)
+    rlen = np.ones(nsite, np.int64)
+    alt = []
+    dl = rng.integers(1, max_indel + 1, nsite)
+    ins = rng.random(nsite) < 0.5
+    for i in range(nsite):
+        r = ref[pos[i]]
+        if kind[i] == 0:
+            alt.append(np.array([BASES[(int(np.flatnonzero(BASES == r)[0]) + rng.integers(1, 4)) % 4]], np.uint8))
+        elif ins[i]:
+            alt.append(np.concatenate([[r], BASES[rng.integers(0, 4, dl[i])]]).astype(np.uint8))
+        else:
+            rlen[i] = 1 + dl[i]
+            alt.append(np.array([r], np.uint8))
+    # --- the two haplotypes and their piece maps back to the reference
+    haps, maps = [], []
+    for h in (0, 1):
+        sel = np.flatnonzero(phase == h)
+        pieces = []
+        seq = _apply_sites(ref, 0, L, pos[sel], rlen[sel], [alt[i] for i in sel], pieces)
+        ref_start = np.array([p[0] for p in pieces], np.int64)
+        plen = np.array([p[1] for p in pieces], np.int64)
+        is_ref = np.array([p[2] for p in pieces], bool)
+        hap_start = np.concatenate([[0], np.cumsum(plen)[:-1]])
+        haps.append(seq)
+        maps.append((hap_start, ref_start, is_ref))
+    # --- reads
+    rs_all, seqs, quals = [], [], []
+    nper = int(round(depth * L / (2 * read_len)))
+    for h in (0, 1):
+        seq = haps[h]
+        hs, rstart, isref = maps[h]
+        st = np.sort(rng.integers(0, len(seq) - read_len + 1, nper))
+        piece = np.searchsorted(hs, st, side="right") - 1
+        rs = np.where(isref[piece], rstart[piece] + (st - hs[piece]), rstart[piece])
+        rs_all.append(rs)
+        win = np.lib.stride_tricks.sliding_window_view(seq, read_len)
+        for c0 in range(0, nper, 1 << 19):
+            idx = st[c0:c0 + (1 << 19)]
+            r = win[idx].copy()
+            q = np.where(rng.random(r.shape, dtype=np.float32) < 0.85,
+                         rng.integers(30, 41, r.shape, dtype=np.uint8),
+                         rng.integers(10, 30, r.shape, dtype=np.uint8))
+            err = rng.random(r.shape, dtype=np.float32) < (10.0 ** (-q.astype(np.float32) / 10))
+            if err.any():
+                code = np.searchsorted(BASES, r[err])
+                r[err] = BASES[(code + rng.integers(1, 4, code.shape)) % 4]
+            seqs.append(r)
+            quals.append(q)
+    rs = np.concatenate(rs_all)
+    rseq = np.concatenate(seqs)
+    rqual = np.concatenate(quals)
+    nreads_total = len(rs)
+    read_mq = np.where(rng.random(nreads_total) < 0.95, 60, rng.integers(20, 60, nreads_total)).astype(np.uint8)
+    o = np.argsort(rs, kind="stable")
+    rs, rseq, rqual, read_mq = rs[o], rseq[o], rqual[o], read_mq[o]
+    # --- windows: reads overlapping [w*window, (w+1)*window)
+    nwin = (L - 2 * (padding + 16)) // window
+    w0 = padding + 16
+    wstart = w0 + np.arange(nwin, dtype=np.int64) * window
+    first = np.searchsorted(rs, wstart - read_len + 1, side="left")
+    last = np.searchsorted(rs, wstart + window, side="left")
+    counts = last - first
+    region_reads = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    ridx = np.concatenate([np.arange(a, b) for a, b in zip(first, last)]) if nwin else np.zeros(0, np.int64)
+    read_bytes = rseq[ridx].reshape(-1)
+    qual_bytes = rqual[ridx].reshape(-1)
+    read_off = np.arange(len(ridx) + 1, dtype=np.int64) * read_len
+    mq = read_mq[ridx]
+    # --- candidate haplotypes per window
+    site_w = np.searchsorted(wstart, pos, side="right") - 1
+    hb, hlen, nh, true_pair = [], [], [], []
+    nfalse = rng.poisson(false_sites, nwin)
+    kmax = int(math.floor(math.log2(max_haps)))
+    for w in range(nwin):
+        s0, s1 = int(wstart[w]), int(wstart[w] + window)
+        tsel = np.flatnonzero((site_w == w) & (pos + rlen <= s1 + padding))
+        cpos = [int(x) for x in pos[tsel]]
+        crl = [int(x) for x in rlen[tsel]]
+        calt = [alt[i] for i in tsel]
+        cph = [int(phase[i]) for i in tsel]
+        for _ in range(int(nfalse[w])):      # candidate SNVs away from every other site
+            p = int(rng.integers(s0, s1))
+            if all(abs(p - x) >= 12 for x in cpos):
+                r = ref[p]
+                cpos.append(p)
+                crl.append(1)
+                calt.append(np.array([BASES[(int(np.flatnonzero(BASES == r)[0]) + rng.integers(1, 4)) % 4]],
+                                     np.uint8))
+                cph.append(-1)
+        while len(cpos) > kmax:              # drop candidates first, then the last true site
+            drop = max((i for i in range(len(cpos)) if cph[i] < 0), default=len(cpos) - 1)
+            for lst in (cpos, crl, calt, cph):
+                lst.pop(drop)
+        o2 = np.argsort(cpos, kind="stable")
+        cpos = [cpos[i] for i in o2]
+        crl = [crl[i] for i in o2]
+        calt = [calt[i] for i in o2]
+        cph = [cph[i] for i in o2]
+        k = len(cpos)
+        for mask in range(1 << k):
+            sel = [i for i in range(k) if mask >> i & 1]
+            hseq = _apply_sites(ref, s0 - padding, s1 + padding, [cpos[i] for i in sel], [crl[i] for i in sel],
+                                [calt[i] for i in sel])
+            hb.append(hseq)
+            hlen.append(len(hseq))
+        nh.append(1 << k)
+        m0 = sum(1 << i for i in range(k) if cph[i] == 0)
+        m1 = sum(1 << i for i in range(k) if cph[i] == 1)
+        a, b = min(m0, m1), max(m0, m1)
+        true_pair.append(b * (b + 1) // 2 + a)
+    hap_bytes = np.concatenate(hb) if hb else np.zeros(0, np.uint8)
+    hap_off = np.concatenate([[0], np.cumsum(hlen)]).astype(np.int64)
+    region_haps = np.concatenate([[0], np.cumsum(nh)]).astype(np.int64)
+    batch = RegionBatch(read_bytes, qual_bytes, read_off, hap_bytes, hap_off, region_reads, region_haps,
+                        gap=gap, ext=ext)
+    info = {"seed": seed, "bases": L, "depth": depth, "read_len": read_len, "window": window,
+            "padding": padding, "het_snvs": int(np.sum(kind == 0)), "het_indels": int(np.sum(kind == 1)),
+            "sample_reads": int(nreads_total), "regions": int(nwin), "batch_reads": int(len(ridx)),
+            "haplotypes": int(region_haps[-1]), "max_haps": int(max(nh) if nh else 0)}
+    return ChromosomeSample(batch, mq, np.array(true_pair, np.int64), wstart, info)
