@@ -227,8 +227,9 @@ typedef struct vd_geno_stats {
 /* The diploid likelihood of every region of a batch of matrices (device
  * pointers; the batched form of vd_mixture_general with triangular order or
  * square).  keep (optional, per row): rows with keep == 0 are left out.
- * stream: a hipStream_t or NULL for the library's stream.  stats (optional)
- * synchronises and fills the counters and times. */
+ * stream: a hipStream_t to enqueue on (the call returns at once), or NULL for
+ * the library's own stream (the call returns when the work is done).  stats
+ * (optional) synchronises and fills the counters and times. */
 int vd_mixture_regions_device(const vd_geno_batch *batch, const double *lh, const signed char *keep,
                               double prior, int triangular, double *gl, void *stream,
                               vd_geno_stats *stats);
@@ -239,7 +240,8 @@ int vd_mixture_regions_device(const vd_geno_batch *batch, const double *lh, cons
  * min_certainty dropped (whole_read.py:37 uses ln 3; NaN is kept), molecule
  * rows exp(log(max(1e-300, v)) - max), then the triangular diploid likelihood
  * with the given prior (whole_read.py:49 uses 1e-3) into gl.  values
- * (rows x columns, like ll) and certainty (per row) are optional outputs. */
+ * (rows x columns, like ll) and certainty (per row) are optional outputs.
+ * stream and stats as for vd_mixture_regions_device. */
 int vd_genotype_device(const vd_geno_batch *batch, const double *ll, const unsigned char *mq,
                        double min_certainty, double prior, double *gl, double *values, double *certainty,
                        void *stream, vd_geno_stats *stats);

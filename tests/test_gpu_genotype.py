@@ -179,6 +179,7 @@ def test_genotype_edge_regions():
     gl_off = engine.gl_offsets(ncols)
     exp, bad = oracle.genotype_regions(row_off, ncols.astype(np.int32), mat_off, ll, mq, gl_off)
     assert bad == 1
+    # NULL stream: the library's own stream, and the call returns when the work is done
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     d_row, d_col, d_mat, d_gl = t(row_off), t(col_off), t(mat_off), t(gl_off)
     d_ll, d_mq = t(ll), t(mq)
@@ -219,7 +220,9 @@ def test_mixture_regions_device_square_with_keep():
         for prior in (-1.0, 0.0, 0.01):
             code = _lib.lib().vd_mixture_regions_device(ctypes.byref(gb), ctypes.c_void_p(lh.data_ptr()),
                                                         ctypes.c_void_p(keep.data_ptr()), prior, tri,
-                                                        ctypes.c_void_p(out.data_ptr()), None, None)
+                                                        ctypes.c_void_p(out.data_ptr()),
+                                                        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                                        None)
             _lib.check(code, "vd_mixture_regions_device")
             got = out.cpu().numpy()
             for k, (m, kp) in enumerate(zip(mats, keeps)):

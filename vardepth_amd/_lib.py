@@ -88,6 +88,14 @@ def lib() -> ctypes.CDLL:
     if not LIB_PATH.exists():
         raise VdhmmError("libvdhmm.so is not built: run `python -m vardepth_amd.build` "
                          "(the HIP engine has no CPU fallback)")
+    # One HIP runtime per process: torch carries its own libamdhip64, and the
+    # library's dependency on libamdhip64.so.7 binds to whichever copy is loaded
+    # first.  Loading torch's first lets torch and the engine share it (loading
+    # the system copy first leaves torch unable to see the device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(str(LIB_PATH))
     L.pairhmm_pairs.argtypes = [_c_char_pp, _u8pp, _ip, _c_char_pp, _ip, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, _dp]
