@@ -286,7 +286,8 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
     res = {}
     for depth in depths:
         t0 = time.perf_counter()
-        cs = workloads.chromosome_sample(mb=mb, depth=depth)
+        cs = workloads.chromosome_sample(mb=mb, depth=depth,
+                                         progress=lambda m: print(m, file=sys.stderr, flush=True))
         gen_s = time.perf_counter() - t0
         b = cs.batch
         cells = b.cells()
@@ -304,9 +305,17 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
         fst = dev.forward(mode=mode, stats=True)
-        gl, gst = dev.genotype(mq, stats=True)
+        gl, _, cert, gst = dev.genotype(mq, stats=True, values=True)
         g = gl.cpu().numpy()
         ll = dev.result()
+        # k_mixture's algorithmic work: every kept row x pair x mixture term, 4 FP64
+        # flops each (two products and a sum or max forming the mixture value, the
+        # running product); the clean-only / no-mixture priors are not used here
+        kept = (~(cert.cpu().numpy() < engine.GENO_MIN_CERTAINTY)).astype(np.int64)
+        csum = np.concatenate([[0], np.cumsum(kept)])
+        kept_k = csum[b.region_reads[1:]] - csum[b.region_reads[:-1]]
+        terms = int(np.sum(kept_k * (nh * (nh + 1) // 2) * (7 + 4 * nh)))
+        mix_tflops = terms * 4 / (gst["mixture_ms"] * 1e-3) / 1e12 if gst["mixture_ms"] > 0 else 0.0
         calls = np.array([int(np.argmax(g[gl_off[k]:gl_off[k + 1]])) for k in range(b.nregions)])
         var = nh >= 2
         # CPU chain on a time-bounded random sample of regions
@@ -349,6 +358,9 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
             "forward_ms": round(fst["device_ms"], 3), "molecule_ms": round(gst["molecule_ms"], 3),
             "mixture_ms": round(gst["mixture_ms"], 3), "mixture_units": gst["units"],
             "kept_rows": gst["kept_rows"], "invalid_regions": gst["invalid"],
+            "mixture_roofline": {"bound": "valu-fp64", "terms": terms, "flops_per_term": 4,
+                                 "achieved": round(mix_tflops, 3), "peak": 78.6, "unit": "TFLOP/s",
+                                 "frac": round(mix_tflops / 78.6, 4)},
             "truth_concordance": round(float(np.mean(calls[var] == cs.true_pair[var])), 4) if var.any() else None,
             "cpu": {"gcups": round(cpu_gcups, 3) if cpu_gcups else None, "threads": threads,
                     "kind": "port", "sample_regions": int(done), "sample_cells": int(cpu_cells),
@@ -402,6 +414,11 @@ def main():
     from vardepth_amd import engine
 
     if args.only_genotype:
+        if args.copy_calib:
+            src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+            torch.empty_like(src).copy_(src)
+            torch.cuda.synchronize()
+            del src
         print(json.dumps({"genotype": genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")],
                                                      args.mode, max(1, min(args.steps, 3)), args.geno_cpu_seconds,
                                                      min(available_cpus(), 16))}), flush=True)

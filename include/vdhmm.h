@@ -227,9 +227,10 @@ typedef struct vd_geno_stats {
 /* The diploid likelihood of every region of a batch of matrices (device
  * pointers; the batched form of vd_mixture_general with triangular order or
  * square).  keep (optional, per row): rows with keep == 0 are left out.
- * stream: a hipStream_t to enqueue on (the call returns at once), or NULL for
- * the library's own stream (the call returns when the work is done).  stats
- * (optional) synchronises and fills the counters and times. */
+ * stream: a hipStream_t, NULL = the default stream, as for vd_forward_device
+ * (so work the caller enqueued on the same stream, such as the forward that
+ * wrote the likelihoods, is complete before it runs); the call only enqueues.
+ * stats (optional) synchronises the stream and fills the counters and times. */
 int vd_mixture_regions_device(const vd_geno_batch *batch, const double *lh, const signed char *keep,
                               double prior, int triangular, double *gl, void *stream,
                               vd_geno_stats *stats);

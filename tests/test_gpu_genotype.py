@@ -179,7 +179,7 @@ def test_genotype_edge_regions():
     gl_off = engine.gl_offsets(ncols)
     exp, bad = oracle.genotype_regions(row_off, ncols.astype(np.int32), mat_off, ll, mq, gl_off)
     assert bad == 1
-    # NULL stream: the library's own stream, and the call returns when the work is done
+    # NULL stream: the default stream, ordered after torch's copies above; stats synchronises
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     d_row, d_col, d_mat, d_gl = t(row_off), t(col_off), t(mat_off), t(gl_off)
     d_ll, d_mq = t(ll), t(mq)
@@ -251,3 +251,21 @@ def test_genotype_graph_replay(sample):
         assert torch.equal(out, ref)
         dev.forward(mode=0)           # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_genotype_follows_forward_on_default_stream(sample):
+    """Forward and genotype enqueued back to back on torch's default stream
+    (handle NULL) with no synchronisation in between: the genotype kernels run
+    after the forward has written every likelihood (the buffer starts as NaN,
+    so reading it early would show)."""
+    import torch
+    dev = engine.DeviceBatch(sample.batch)
+    mq = torch.from_numpy(sample.mq).cuda()
+    dev.forward(mode=0)
+    ref = dev.genotype(mq, stats=True)[0].clone()
+    for _ in range(3):
+        dev.out.fill_(float("nan"))
+        dev.forward(mode=0)
+        got = dev.genotype(mq)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref)

@@ -647,13 +647,11 @@ int geno_stats(hipStream_t st, vd_geno_stats* s, bool molecule) {
 
 int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t* keep, double prior, int tri,
                           double* gl, hipStream_t st, vd_geno_stats* s) {
-  const bool sync = st == nullptr;  // the library's own stream: return when done
   if (int rc = check_geno_batch(b)) return rc;
   if (int rc = check_prior(prior, true)) return rc;
   if (b->nregions > 0 && ((!lh && b->mat_size > 0) || (!gl && b->gl_size > 0)))
     return fail("vdhmm: null genotype matrix or output");
   if (int rc = engine_init()) return rc;
-  if (!st) st = g.own_stream;
   if (int rc = geno_init(st)) return rc;
   const vd::GenoDev d = geno_dev(b);
   HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
@@ -663,21 +661,18 @@ int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t
   HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, gg.units.as<int64_t>(), gg.map.as<int32_t>(), gl,
                             gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
-  if (sync) HIPCHK(hipStreamSynchronize(st));
   return geno_stats(st, s, false);
 }
 
 int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8_t* mq, double min_cert,
                            double prior, double* gl, double* values, double* cert, hipStream_t st,
                            vd_geno_stats* s) {
-  const bool sync = st == nullptr;  // the library's own stream: return when done
   if (int rc = check_geno_batch(b)) return rc;
   if (int rc = check_prior(prior, true)) return rc;
   if (std::isnan(min_cert)) return fail("vdhmm: min_certainty is NaN");
   if (b->nregions > 0 && ((!ll && b->mat_size > 0) || (!mq && b->nrows > 0) || (!gl && b->gl_size > 0)))
     return fail("vdhmm: null genotype input or output");
   if (int rc = engine_init()) return rc;
-  if (!st) st = g.own_stream;
   if (int rc = geno_init(st)) return rc;
   const vd::GenoDev d = geno_dev(b);
   HIPCHK(gg.mol.ensure(sizeof(double) * std::max<int64_t>(b->mat_size, 1)));
@@ -692,7 +687,6 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   HIPCHK(vd::launch_mixture(d, gg.mol.as<double>(), gg.keep.as<int8_t>(), prior, 1, gg.units.as<int64_t>(),
                             gg.map.as<int32_t>(), gl, gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
-  if (sync) HIPCHK(hipStreamSynchronize(st));
   return geno_stats(st, s, true);
 }
 

@@ -16,6 +16,12 @@ runs, so FETCH_SIZE / WRITE_SIZE can be scaled to bytes per dispatch.
     python -m vardepth_amd.profiling traffic FETCH_DB WRITE_DB [CALIB_DB]
         HBM bytes per dispatch per kernel, each counter scaled by what it
         reads for the 256 MiB copy (taken from CALIB_DB when given)
+    python -m vardepth_amd.profiling families KT_DB [FETCH_DB WRITE_DB]
+        kernels grouped by family (the template instances of one kernel, e.g.
+        the per-class k_fwd32<A, R, U>): variants, dispatches per variant, the
+        sum over variants of the average duration (one pass of the batch runs
+        each variant once) and, with the PMC databases, the summed calibrated
+        bytes per pass
 """
 from __future__ import annotations
 
@@ -100,7 +106,39 @@ def traffic_table(fetch_db: str, write_db: str, calib_db: str | None = None):
     return sorted(out, key=lambda r: -(r[2] + r[3])), cf, cw
 
 
+def family_of(name: str) -> str:
+    base = name.split("(")[0].split("<")[0]
+    return base.replace("void ", "").split("::")[-1].strip()
+
+
+def family_table(kt_db: str, fetch_db: str | None = None, write_db: str | None = None):
+    fam = {}
+    for name, n, avg, _, _, total in kernels(kt_db):
+        e = fam.setdefault(family_of(name), {"variants": 0, "calls": 0, "avg_sum": 0.0, "total": 0.0,
+                                             "fetch": 0.0, "write": 0.0})
+        e["variants"] += 1
+        e["calls"] += n
+        e["avg_sum"] += avg
+        e["total"] += total
+    if fetch_db and write_db:
+        rows, _, _ = traffic_table(fetch_db, write_db)
+        for name, _, fb, wb in rows:
+            e = fam.get(family_of(name))
+            if e is not None:
+                e["fetch"] += fb
+                e["write"] += wb
+    return sorted(fam.items(), key=lambda kv: -kv[1]["total"])
+
+
 def main(argv):
+    if len(argv) >= 2 and argv[0] == "families":
+        rows = family_table(*argv[1:4])
+        print("%-22s %8s %7s %14s %12s %14s %14s" % ("family", "variants", "calls", "sum avg ms", "total ms",
+                                                     "fetch B/pass", "write B/pass"))
+        for name, e in rows:
+            print("%-22s %8d %7d %14.3f %12.3f %14.4g %14.4g" % (name, e["variants"], e["calls"], e["avg_sum"],
+                                                                e["total"], e["fetch"], e["write"]))
+        return 0
     if len(argv) >= 2 and argv[0] == "sq":
         print("%-60s %5s %10s %6s %12s %8s %6s %6s" % ("kernel", "calls", "ms", "GHz", "VALU", "per-cyc", "util",
                                                        "waves"))

@@ -438,7 +438,8 @@ def _apply_sites(ref: np.ndarray, lo: int, hi: int, pos, rlen, alt, pieces_out=N
 def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, read_len: int = 150,
                       window: int = 400, padding: int = 200, snv_rate: float = 1 / 1500,
                       indel_rate: float = 1 / 10000, max_indel: int = 10, false_sites: float = 1.0,
-                      max_haps: int = 64, gap: float = GAP_Q45, ext: float = EXT_Q10) -> ChromosomeSample:
+                      max_haps: int = 64, gap: float = GAP_Q45, ext: float = EXT_Q10,
+                      progress=None) -> ChromosomeSample:
     """Seeded chromosome-scale diploid sample.
 
     * reference: ``mb`` million uniform random bases;
@@ -453,6 +454,7 @@ def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, 
       haplotypes are ref[start - padding : end + padding] with every subset of
       the sites applied (subset mask order, so haplotype 0 is the reference),
       trimmed to at most ``max_haps`` by dropping candidate sites first;
+    * ``progress`` (optional callable) receives a short message per stage;
     * each region holds the reads overlapping its window, in start order
       (whole_read.py:16-27 fetches the overlapping reads; with reads shorter
       than the padding every one lies inside the padded window).
@@ -498,6 +500,8 @@ def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, 
         hap_start = np.concatenate([[0], np.cumsum(plen)[:-1]])
         haps.append(seq)
         maps.append((hap_start, ref_start, is_ref))
+    if progress:
+        progress("chromosome sample: %d sites on two haplotypes" % nsite)
     # --- reads
     rs_all, seqs, quals = [], [], []
     nper = int(round(depth * L / (2 * read_len)))
@@ -521,6 +525,8 @@ def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, 
                 r[err] = BASES[(code + rng.integers(1, 4, code.shape)) % 4]
             seqs.append(r)
             quals.append(q)
+    if progress:
+        progress("chromosome sample: %d reads" % (2 * nper))
     rs = np.concatenate(rs_all)
     rseq = np.concatenate(seqs)
     rqual = np.concatenate(quals)
@@ -547,6 +553,8 @@ def chromosome_sample(seed: int = 41005, mb: float = 60.0, depth: float = 30.0, 
     nfalse = rng.poisson(false_sites, nwin)
     kmax = int(math.floor(math.log2(max_haps)))
     for w in range(nwin):
+        if progress and w and w % 25000 == 0:
+            progress("chromosome sample: haplotypes of %d / %d regions" % (w, nwin))
         s0, s1 = int(wstart[w]), int(wstart[w] + window)
         tsel = np.flatnonzero((site_w == w) & (pos + rlen <= s1 + padding))
         cpos = [int(x) for x in pos[tsel]]
