@@ -247,6 +247,19 @@ int vd_genotype_device(const vd_geno_batch *batch, const double *ll, const unsig
                        double min_certainty, double prior, double *gl, double *values, double *certainty,
                        void *stream, vd_geno_stats *stats);
 
+/* Near-tie guard of partial_anchor.py:119-133 on a genotype batch (device
+ * pointers): per region, gaps[2k] = largest - second largest of gl, gaps[2k+1]
+ * the same of gl + log(prior) with the uniform haplotype prior of
+ * whole_read.py:51 (1 for a == b, 2 otherwise, over n^2); flags[k] = 1 when
+ * the smaller gap is <= margin_scale * max(1, molecules), molecules = rows
+ * whose certainty (as written by vd_genotype_device) is not below
+ * min_certainty.  partial_anchor.py uses margin_scale 1e-4 and recomputes a
+ * flagged locus in FP64.  Regions with fewer than two pairs are not flagged.
+ * nflagged (optional) synchronises and returns the count. */
+int vd_genotype_guard(const vd_geno_batch *batch, const double *gl, const double *certainty,
+                      double min_certainty, double margin_scale, signed char *flags, double *gaps,
+                      void *stream, int64_t *nflagged);
+
 /* ---- baseline ----------------------------------------------------------- */
 
 /* The plain GPU forward: the first max_pairs pairs of the batch (region

@@ -269,3 +269,54 @@ def test_genotype_follows_forward_on_default_stream(sample):
         got = dev.genotype(mq)
         torch.cuda.synchronize()
         assert torch.equal(got, ref)
+
+
+def test_near_tie_guard_recomputes_in_fp64(sample):
+    """Regions given a duplicate candidate haplotype have exactly tied
+    genotypes: the guard flags them (with the rule of partial_anchor.py:119-133
+    applied to the device's likelihoods) and replaces their outputs by the chain
+    on all-FP64 likelihoods; every other region keeps its mixed-precision
+    result."""
+    import torch
+    src = sample.batch
+    regions, mqs = [], []
+    for k in range(src.nregions):
+        reads, quals, haps = src.region(k)
+        if k % 5 == 0 and haps:
+            haps = haps + [haps[0]]
+        regions.append((reads, quals, haps))
+        mqs.append(sample.mq[src.region_reads[k]:src.region_reads[k + 1]])
+    b = workloads.from_regions(regions, src.gap, src.ext)
+    mq_h = np.concatenate(mqs)
+    mq = torch.from_numpy(mq_h).cuda()
+    dev = engine.DeviceBatch(b)
+    dev.forward(mode=0)
+    plain, _, cert = dev.genotype(mq, values=True)
+    plain = plain.clone()
+    cert = cert.cpu().numpy()
+    guarded, report = dev.genotype_guarded(mq, mode=0)
+    g0 = plain.cpu().numpy()
+    g1 = guarded.cpu().numpy()
+    nh = np.diff(b.region_haps)
+    off = engine.gl_offsets(nh)
+    expected = []
+    for k in range(b.nregions):
+        gl = g0[off[k]:off[k + 1]]
+        if len(gl) < 2:
+            continue
+        n = int(nh[k])
+        prior = np.array([1.0 if a == c else 2.0 for c in range(n) for a in range(c + 1)])
+        lp = gl + np.log(prior / prior.sum())
+        molecules = int(np.sum(~(cert[b.region_reads[k]:b.region_reads[k + 1]] < math.log(3))))
+        gaps = [float(np.diff(np.partition(v, -2)[-2:])[0]) for v in (gl, lp)]
+        if min(gaps) <= 1e-4 * max(1, molecules):
+            expected.append(k)
+    flagged = [r["region"] for r in report["regions"]]
+    assert flagged == expected
+    assert flagged and set(flagged) & set(range(0, b.nregions, 5))   # the planted ties are found
+    full = engine.DeviceBatch(b)
+    full.forward(mode=0, precision="float64")
+    g64 = full.genotype(mq).cpu().numpy()
+    for k in range(b.nregions):
+        sl = slice(off[k], off[k + 1])
+        np.testing.assert_array_equal(g1[sl], g64[sl] if k in set(flagged) else g0[sl])

@@ -30,7 +30,8 @@ SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch
            "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
            "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_tuning_get",
            "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_mixture_general", "vd_mixture_two",
-           "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_last_error", "vd_device")
+           "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_genotype_guard",
+           "vd_last_error", "vd_device")
 
 
 class VdParams(ctypes.Structure):
@@ -158,6 +159,10 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(VdGenoStats)]
     L.vd_genotype_device.restype = ctypes.c_int
+    L.vd_genotype_guard.argtypes = [ctypes.POINTER(VdGenoBatch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.POINTER(ctypes.c_int64)]
+    L.vd_genotype_guard.restype = ctypes.c_int
     L.vd_last_error.argtypes = []
     L.vd_last_error.restype = ctypes.c_char_p
     L.vd_device.argtypes = []

@@ -690,6 +690,25 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   return geno_stats(st, s, true);
 }
 
+int guard_device_locked(const vd_geno_batch* b, const double* gl, const double* cert, double min_cert,
+                        double margin_scale, int8_t* flags, double* gaps, hipStream_t st, int64_t* nflagged) {
+  if (int rc = check_geno_batch(b)) return rc;
+  if (std::isnan(min_cert) || !(margin_scale >= 0)) return fail("vdhmm: invalid guard margin or certainty bound");
+  if (b->nregions > 0 && (!gl || !cert || !flags || !gaps)) return fail("vdhmm: null guard argument");
+  if (int rc = engine_init()) return rc;
+  if (int rc = geno_init(st)) return rc;
+  if (int rc = geno_zero_ctr(st)) return rc;
+  HIPCHK(vd::launch_guard(geno_dev(b), gl, cert, min_cert, margin_scale, flags, gaps,
+                          gg.ctr.as<unsigned long long>(), st));
+  if (nflagged) {
+    HIPCHK(hipStreamSynchronize(st));
+    unsigned long long c[vd::GCTR_COUNT];
+    HIPCHK(hipMemcpy(c, gg.ctr.p, sizeof(c), hipMemcpyDeviceToHost));
+    *nflagged = (int64_t)c[vd::GCTR_GUARD];
+  }
+  return 0;
+}
+
 // one region of rows x n staged from host memory
 int stage_one_region(int rows, int n, int64_t pairs, vd_geno_batch* b, hipStream_t st) {
   const int64_t off[8] = {0, rows, 0, n, 0, 0, 0, 0};  // row_off[2], col_off[2], mat_off, gl_off
@@ -980,6 +999,13 @@ int vd_genotype_device(const vd_geno_batch* batch, const double* ll, const unsig
   if (stats && !rc)
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+int vd_genotype_guard(const vd_geno_batch* batch, const double* gl, const double* certainty, double min_certainty,
+                      double margin_scale, signed char* flags, double* gaps, void* stream, int64_t* nflagged) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return guard_device_locked(batch, gl, certainty, min_certainty, margin_scale, (int8_t*)flags, gaps,
+                             (hipStream_t)stream, nflagged);
 }
 
 int vd_tuning_get(vd_tuning* t) {

@@ -302,6 +302,46 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
   }
 }
 
+// Near-tie guard (partial_anchor.py:119-133): per region, the gaps between the
+// two largest values of gl and of lp = gl + log(prior), prior = 1 (a == b) or 2
+// (a != b) over n^2 (whole_read.py:51), against margin_scale * max(1, molecules),
+// molecules = rows whose certainty is not below min_cert.  Regions with fewer
+// than two pairs are never flagged.
+__global__ void k_guard(GenoDev g, const double* __restrict__ gl, const double* __restrict__ cert, double min_cert,
+                        double margin_scale, int8_t* __restrict__ flag, double* __restrict__ gaps,
+                        unsigned long long* __restrict__ ctr) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < g.nregions;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = g.col_off[k + 1] - g.col_off[k];
+    const int64_t pairs = n * (n + 1) / 2;
+    const int64_t r0 = g.row_off[k], r1 = g.row_off[k + 1];
+    const int64_t go = g.gl_off[k];
+    flag[k] = 0;
+    gaps[2 * k] = gaps[2 * k + 1] = __builtin_inf();
+    if (pairs < 2 || go < 0 || go + pairs > g.gl_size || r0 < 0 || r1 > g.nrows) continue;
+    int64_t mol = 0;
+    for (int64_t r = r0; r < r1; ++r) mol += !(cert[r] < min_cert);
+    const double lw_hom = log(1.0 / (double)(n * n)), lw_het = log(2.0 / (double)(n * n));
+    double a1 = -__builtin_inf(), a2 = -__builtin_inf(), b1 = -__builtin_inf(), b2 = -__builtin_inf();
+    int64_t q = 0;
+    for (int64_t bb = 0; bb < n; ++bb)
+      for (int64_t aa = 0; aa <= bb; ++aa, ++q) {
+        const double v = gl[go + q];
+        const double w = v + (aa == bb ? lw_hom : lw_het);
+        if (v > a1) { a2 = a1; a1 = v; } else if (v > a2) { a2 = v; }
+        if (w > b1) { b2 = b1; b1 = w; } else if (w > b2) { b2 = w; }
+      }
+    const double ga = a1 - a2, gb = b1 - b2;
+    gaps[2 * k] = ga;
+    gaps[2 * k + 1] = gb;
+    const double margin = margin_scale * (mol > 1 ? (double)mol : 1.0);
+    if ((ga < gb ? ga : gb) <= margin) {
+      flag[k] = 1;
+      atomicAdd(&ctr[GCTR_GUARD], 1ull);
+    }
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -310,6 +350,14 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, double min_cert, double margin_scale,
+                        int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st) {
+  if (g.nregions <= 0) return hipSuccess;
+  const int blocks = (int)((g.nregions + 255) / 256 < 4096 ? (g.nregions + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k_guard, dim3(blocks), dim3(256), 0, st, g, gl, cert, min_cert, margin_scale, flag, gaps, ctr);
   return hipGetLastError();
 }
 

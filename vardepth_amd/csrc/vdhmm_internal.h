@@ -126,7 +126,8 @@ enum {
   GCTR_INVALID = 1,  // regions whose matrix held a non-finite or negative entry
   GCTR_KEPT = 2,     // rows that passed the certainty filter
   GCTR_LAYOUT = 3,   // regions skipped: block outside the matrix / output bounds, or unit map full
-  GCTR_COUNT = 4
+  GCTR_GUARD = 4,    // regions flagged by the near-tie guard
+  GCTR_COUNT = 8
 };
 
 struct LaunchCfg {
@@ -159,6 +160,8 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
                        double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, double min_cert, double margin_scale,
+                        int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st);
 hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq, const double* mq_err,
                            double min_cert, double* values, double* cert, double* mol, int8_t* keep,
                            unsigned long long* ctr, int cus, hipStream_t st);

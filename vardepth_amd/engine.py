@@ -104,6 +104,7 @@ def forward_plain(batch: RegionBatch, mode=0, err=None, max_pairs=None):
 
 GENO_MIN_CERTAINTY = float(np.log(3))   # whole_read.py:37
 GENO_PRIOR = 1e-3                        # whole_read.py:49
+GUARD_MARGIN = 1e-4                      # partial_anchor.py:125: 1e-4 * max(1, molecules)
 
 
 def gl_offsets(ncols, triangular=True) -> np.ndarray:
@@ -223,6 +224,55 @@ class DeviceBatch:
         if stats:
             res.append(st.as_dict())
         return res[0] if len(res) == 1 else tuple(res)
+
+    def genotype_guarded(self, mq, mode=0, min_certainty=GENO_MIN_CERTAINTY, prior=GENO_PRIOR,
+                         margin_scale=GUARD_MARGIN, stream=None):
+        """The genotype chain with the near-tie guard of partial_anchor.py:119-133.
+
+        After :meth:`genotype` on the mixed-precision likelihoods in self.out
+        (run ``forward(mode=mode)`` first), every region whose two best
+        genotypes -- by likelihood, or by likelihood plus the uniform
+        haplotype prior -- are within ``margin_scale * max(1, molecules)`` is
+        recomputed from an all-FP64 forward of its reads and haplotypes, and
+        its outputs replaced.  Returns (gl device tensor, report): the report
+        lists the flagged regions with their observed gaps and margins."""
+        torch = self.torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        gl, _, cert = self.genotype(mq, min_certainty, prior, stream=s, values=True)
+        nreg = self.batch.nregions
+        flags = torch.empty(max(nreg, 1), dtype=torch.int8, device=self.out.device)
+        gaps = torch.empty(2 * max(nreg, 1), dtype=torch.float64, device=self.out.device)
+        nflag = ctypes.c_int64()
+        code = _lib.lib().vd_genotype_guard(ctypes.byref(self.gb), ctypes.c_void_p(gl.data_ptr()),
+                                            ctypes.c_void_p(cert.data_ptr()), float(min_certainty),
+                                            float(margin_scale), ctypes.c_void_p(flags.data_ptr()),
+                                            ctypes.c_void_p(gaps.data_ptr()), ctypes.c_void_p(s.cuda_stream),
+                                            ctypes.byref(nflag))
+        _lib.check(code, "vd_genotype_guard")
+        report = {"backend": "hip_float64", "reason": "near_tied_likelihood_or_posterior",
+                  "margin_scale": margin_scale, "regions": []}
+        if nflag.value == 0:
+            return gl, report
+        ids = np.flatnonzero(flags[:nreg].cpu().numpy())
+        g = gaps[:2 * nreg].cpu().numpy().reshape(-1, 2)
+        keep = ~(cert.cpu().numpy() < min_certainty)
+        rr = self.batch.region_reads
+        mqh = mq.detach().cpu().numpy().astype(np.uint8)
+        sub = self.batch.subset(ids)
+        dsub = DeviceBatch(sub, device=self.out.device)
+        dsub.forward(mode=mode, precision="float64", stream=s)
+        smq = torch.from_numpy(np.concatenate([mqh[rr[k]:rr[k + 1]] for k in ids])).to(self.out.device)
+        sgl = dsub.genotype(smq, min_certainty, prior, stream=s)
+        off = gl_offsets(np.diff(self.batch.region_haps))
+        soff = gl_offsets(np.diff(sub.region_haps))
+        dst = np.concatenate([np.arange(off[k], off[k + 1]) for k in ids])
+        src = np.concatenate([np.arange(soff[i], soff[i + 1]) for i in range(len(ids))])
+        gl[torch.from_numpy(dst).to(gl.device)] = sgl[torch.from_numpy(src).to(gl.device)]
+        for k in ids:
+            mol = int(np.sum(keep[rr[k]:rr[k + 1]]))
+            report["regions"].append({"region": int(k), "observed_gaps": [float(g[k, 0]), float(g[k, 1])],
+                                      "log_likelihood_margin": margin_scale * max(1, mol)})
+        return gl, report
 
     def result(self) -> np.ndarray:
         return self.out[:self.npairs].cpu().numpy()
