@@ -174,6 +174,14 @@ typedef struct vd_tuning {
 
 int vd_tuning_get(vd_tuning *t);
 int vd_tuning_set(const vd_tuning *t);
+/* Haplotype prefix sharing of the FP32 forward (default on; no counterpart
+ * call in the reference, whose CPU trie_many shares prefixes the same way,
+ * trie_vector.cpp:198-204): each read sweeps its region's haplotypes in byte
+ * order and resumes every haplotype from the column it shares with the
+ * previous one, with results identical to full sweeps.  on = 1 / 0 sets it,
+ * on < 0 only queries; returns the previous setting.  The FP64 path and the
+ * gradients always sweep in full. */
+int vd_prefix_sharing(int on);
 /* When on, calls made with a stats pointer also time each class kernel;
  * vd_class_ms returns the last such times (17 each, index = class). */
 void vd_class_timing(int on);

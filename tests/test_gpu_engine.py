@@ -179,3 +179,46 @@ def test_autotune_keeps_results():
         g = dev.grad(mode=0).cpu().numpy()
         np.testing.assert_allclose(g, g_ref, rtol=1e-12, atol=1e-12)
     autotune.set_tuning(base)
+
+
+def _sharing_cases():
+    rng = np.random.default_rng(77)
+    seq = lambda n: "".join(rng.choice(list("ACGT"), n))
+    cases = [workloads.uneven_regions(seed=7, regions=48), workloads.chromosome_sample(mb=0.05, depth=30).batch]
+    base = seq(300)
+    reads = [base[i:i + L] for i, L in ((0, 40), (10, 150), (50, 250), (120, 100), (5, 17), (200, 96))]
+    quals = [bytes(rng.integers(5, 41, len(r)).astype(np.uint8)) for r in reads]
+    mut = lambda s, i, c: s[:i] + c + s[i + 1:]
+    edge = [
+        # prefix of another, duplicates, an empty haplotype, single-column divergence at the end
+        (reads, quals, [base[:200], base, base, "", mut(base, 299, "A" if base[299] != "A" else "C"), base[:1]]),
+        # many distinct forks: 40 haplotypes diverging at 40 different columns (beyond the 32 kept slots)
+        (reads[:3], quals[:3], [mut(base, 5 * i + 3, "T" if base[5 * i + 3] != "T" else "G") for i in range(40)]),
+        # more haplotypes than the planner ranks (swept without sharing)
+        (reads[:1], quals[:1], [mut(base, i % 300, "ACGT"[i % 4]) for i in range(260)]),
+        # identical haplotypes only
+        (reads[:2], quals[:2], [base[:120]] * 5),
+    ]
+    cases.append(workloads.from_regions(edge))
+    return cases
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_prefix_sharing_matches_full_sweeps_bit_for_bit(mode):
+    L = _lib.lib()
+    prev = L.vd_prefix_sharing(-1)
+    try:
+        for b in _sharing_cases():
+            L.vd_prefix_sharing(0)
+            full, st0 = engine.forward(b, mode=mode, stats=True)
+            L.vd_prefix_sharing(1)
+            shared, st1 = engine.forward(b, mode=mode, stats=True)
+            assert st1["flagged"] == st0["flagged"]
+            # the FP32 sweeps agree bit for bit; pairs rescued in FP64 may differ in
+            # the last bit, because the FP64 kernel tiles a pair by the longest read
+            # of its 4-pair unit and the rescue list is filled in another order
+            differ = np.flatnonzero(~((shared == full) | (np.isnan(shared) & np.isnan(full))))
+            assert len(differ) <= st1["fp64_pairs"], (len(differ), st1["fp64_pairs"])
+            np.testing.assert_allclose(shared[differ], full[differ], rtol=1e-13, atol=0)
+    finally:
+        L.vd_prefix_sharing(prev)

@@ -81,6 +81,11 @@ struct Engine {
   hipEvent_t ev_cls[17] = {};
   double fwd_cls_ms[17] = {}, grad_cls_ms[17] = {};
   int gocc_smax = -1;
+  // haplotype prefix sharing of the FP32 forward
+  bool share = true;
+  int occ32t[2][17] = {};
+  int occt_smax = -1;
+  DevBuf trie_order, trie_start, trie_fork, trie_nfork, trie_ws;
 };
 
 Engine g;
@@ -186,11 +191,12 @@ vd::BatchDev batch_dev(const vd_batch* db) {
                       db->region_reads, db->region_haps, db->out_off, db->nregions, db->nreads, db->nhaps};
 }
 
-vd::PlanDev plan_dev(int64_t npairs, int class_major) {
+vd::PlanDev plan_dev(int64_t npairs, int class_major, int trie = 0) {
   return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
                      g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
                      g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
-                     class_major};
+                     class_major, trie, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
+                     g.trie_fork.as<int32_t>(), g.trie_nfork.as<int32_t>(), g.trie_ws.as<float>()};
 }
 
 int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
@@ -227,8 +233,30 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   }
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
+  const int trie = g.share && !force64 ? 1 : 0;
+  if (trie) {
+    if (g.occt_smax != smax) {
+      for (int a = 0; a < 2; ++a)
+        for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::occupancy_trie(a, r, smax, &g.occ32t[a][r]));
+      g.occt_smax = smax;
+    }
+    size_t ws_floats = 0;
+    for (int r = 1; r <= vd::MAX_R32; ++r) {
+      const int occ = std::max(1, g.occ32t[anchored][r]);
+      const int want = g.tune.fwd_blocks_per_cu[r];
+      blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
+      ws_floats = std::max(ws_floats, (size_t)blocks32[r] * vd::PAIRS_PER_UNIT * vd::TRIE_SLOTS * vd::GROUP *
+                                          ((3 * r + 3 + 3) & ~3));
+    }
+    const int64_t nh = std::max<int64_t>(db->nhaps, 1), nreg = std::max<int64_t>(db->nregions, 1);
+    HIPCHK(g.trie_order.ensure(nh * 4));
+    HIPCHK(g.trie_start.ensure(nh * 4));
+    HIPCHK(g.trie_fork.ensure(nh * 4));
+    HIPCHK(g.trie_nfork.ensure(nreg * 4));
+    HIPCHK(g.trie_ws.ensure(ws_floats * sizeof(float)));
+  }
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs, 1);
+  const vd::PlanDev p = plan_dev(npairs, 1, trie);
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, unroll32, blocks64,
@@ -1027,6 +1055,13 @@ int vd_tuning_set(const vd_tuning* t) {
   g.tune = *t;
   g.tune_init = true;
   return 0;
+}
+
+int vd_prefix_sharing(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int was = g.share ? 1 : 0;
+  if (on >= 0) g.share = on != 0;
+  return was;
 }
 
 void vd_class_timing(int on) {

@@ -91,7 +91,20 @@ struct PlanDev {
   unsigned long long* ctr;  // counters, see CTR_*
   int64_t f64_cap;
   int class_major;          // bucket order: 0 region-major (forward), 1 class-major (gradients)
+  // haplotype prefix sharing of the FP32 forward (trie = 1): a unit is 4 reads
+  // of one (region, class), each group sweeping the region's haplotypes in
+  // lexicographic order and resuming each from the fork column it shares with
+  // its predecessor
+  int trie;
+  int32_t* trie_order;      // [nhaps] region slot h0 + rank -> global haplotype index
+  int32_t* trie_start;      // [nhaps] per rank: resume column s (0 = full sweep)
+  int32_t* trie_fork;       // [nhaps] per region: sorted distinct fork columns at h0 + i, i < trie_nfork[k]
+  int32_t* trie_nfork;      // [nregions]
+  float* trie_ws;           // fork wavefronts: [block][group][TRIE_SLOTS][16 lanes][3R+3 padded to 4]
 };
+
+constexpr int TRIE_SLOTS = 32;  // distinct fork columns kept per region (later forks are swept in full)
+constexpr int TRIE_CHUNK = 8;   // haplotypes per read unit: a region's sorted haplotypes split in chunks
 
 enum {
   CTR_UNIT32 = 0,   // next FP32 unit
@@ -142,6 +155,7 @@ namespace vd {
 size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
+hipError_t occupancy_trie(int anchored, int r, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,

@@ -110,7 +110,14 @@ __global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
     }
     if (threadIdx.x < NCLS) {
       int c = threadIdx.x;
-      int64_t units = (c >= 1 && c <= MAX_R32) ? ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT : 0;
+      // pair units (4 read x haplotype pairs), or with prefix sharing read
+      // units (4 reads, each against every haplotype of the region)
+      const int64_t items = p.trie ? (int64_t)cnt[c] * ((H + TRIE_CHUNK - 1) / TRIE_CHUNK) : (int64_t)cnt[c] * H;
+      int64_t units = (c >= 1 && c <= MAX_R32 && H > 0)
+                          ? (p.trie ? ((int64_t)cnt[c] + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT *
+                                          ((H + TRIE_CHUNK - 1) / TRIE_CHUNK)
+                                    : (items + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT)
+                          : 0;
       p.unit_cnt[bucket_of(c, k, b.nregions, p.class_major)] = units;
     }
     __syncthreads();
@@ -146,6 +153,156 @@ __global__ void k_scan_units(int64_t* __restrict__ a, int64_t n, unsigned long l
   }
 }
 
+// ---- haplotype prefix sharing ---------------------------------------------
+
+constexpr int TRIE_MAXH = 256;  // regions with more haplotypes are swept without sharing
+
+// Common prefix length of haplotypes i and j (global indices) and whether i
+// sorts before j (byte order, a proper prefix first, ties by index), compared
+// 64 bytes per step by the whole wave from byte `from` on (a prefix both are
+// known to share).  All lanes must call it with the same arguments.
+__device__ __forceinline__ int hap_cmp_wave(const BatchDev& b, int64_t i, int64_t j, int64_t from, bool& before) {
+  const int lane = threadIdx.x & 63;
+  const int64_t oi = b.hap_off[i], oj = b.hap_off[j];
+  const int64_t li = b.hap_off[i + 1] - oi, lj = b.hap_off[j + 1] - oj;
+  const int64_t n = li < lj ? li : lj;
+  for (int64_t t0 = from; t0 < n; t0 += 64) {
+    const int64_t t = t0 + lane;
+    const bool diff = t < n && b.hap_bytes[oi + t] != b.hap_bytes[oj + t];
+    const unsigned long long mask = __builtin_amdgcn_ballot_w64(diff);
+    if (mask) {
+      const int64_t at = t0 + __builtin_ctzll(mask);
+      before = b.hap_bytes[oi + at] < b.hap_bytes[oj + at];
+      return (int)at;
+    }
+  }
+  before = li != lj ? li < lj : i < j;
+  return (int)n;
+}
+
+// One wave (block) per region: haplotypes in byte order (trie_order, by binary
+// insertion: O(H log H) wave-wide compares, each starting past the prefix all
+// of the region's haplotypes share), the column each resumes from
+// (trie_start: the common prefix with its predecessor, at most its length - 1,
+// so that at least one column is swept), and the region's distinct fork
+// columns in increasing order (trie_fork), one state slot each; a haplotype
+// whose fork would need a slot beyond TRIE_SLOTS is swept in full.  Any order
+// is correct for the resume (a column j is read from the haplotype that last
+// computed it, and every haplotype in between resumed past j, so by the chain
+// of common prefixes it shares the reader's first j bases); byte order makes
+// neighbours share the most.
+__global__ __launch_bounds__(64) void k_hap_trie(BatchDev b, PlanDev p) {
+  constexpr int PER = (TRIE_MAXH + 63) / 64;  // ranks per lane
+  __shared__ int sorted_sh[TRIE_MAXH];
+  const int lane = threadIdx.x;
+  for (int64_t k = blockIdx.x; k < b.nregions; k += gridDim.x) {
+    const int64_t h0 = b.region_haps[k], H = b.region_haps[k + 1] - h0;
+    const bool share = H <= TRIE_MAXH;
+    if (share) {
+      // prefix shared by every haplotype of the region
+      int64_t common = 0x7fffffffffffffffll;
+      for (int64_t i = 1; i < H; ++i) {
+        bool ib;
+        const int64_t c = hap_cmp_wave(b, h0, h0 + i, 0, ib);
+        common = c < common ? c : common;
+      }
+      if (H <= 1) common = 0;
+      // binary insertion into sorted_sh[0..i)
+      for (int i = 0; i < (int)H; ++i) {
+        int lo = 0, hi = i;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          bool before;
+          hap_cmp_wave(b, h0 + i, h0 + sorted_sh[mid], common, before);
+          if (before) hi = mid; else lo = mid + 1;
+        }
+        // shift sorted_sh[lo..i) up by one, then insert
+        int v[PER];
+#pragma unroll
+        for (int w = 0; w < PER; ++w) {
+          const int q = lane + 64 * w;
+          v[w] = (q > lo && q <= i) ? sorted_sh[q - 1] : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < PER; ++w) {
+          const int q = lane + 64 * w;
+          if (q > lo && q <= i) sorted_sh[q] = v[w];
+        }
+        if (lane == 0) sorted_sh[lo] = i;
+        __syncthreads();
+      }
+#pragma unroll
+      for (int w = 0; w < PER; ++w) {
+        const int q = lane + 64 * w;
+        if (q < H) p.trie_order[h0 + q] = (int32_t)(h0 + sorted_sh[q]);
+      }
+    } else {
+      for (int64_t i = lane; i < H; i += 64) p.trie_order[h0 + i] = (int32_t)(h0 + i);
+    }
+    __syncthreads();
+    __shared__ int st_sh[TRIE_MAXH];
+    __shared__ int first_sh[TRIE_MAXH];
+    for (int64_t q = 0; q < H; ++q) {  // start columns, one wave-wide compare each
+      int st = 0;
+      if (share && q > 0) {
+        const int64_t a = p.trie_order[h0 + q - 1], c = p.trie_order[h0 + q];
+        const int m = (int)(b.hap_off[c + 1] - b.hap_off[c]);
+        bool ib;
+        st = hap_cmp_wave(b, a, c, 0, ib);
+        st = st < m - 1 ? st : m - 1;
+        st = st > 0 ? st : 0;
+      }
+      if (lane == 0) {
+        p.trie_start[h0 + q] = st;
+        if (q < TRIE_MAXH) st_sh[q] = st;
+      }
+    }
+    __syncthreads();
+    const int hs = share ? (int)H : 0;  // ranks with a start column (all zero without sharing)
+    // first occurrence of each distinct fork column, then the slot of each rank:
+    // the number of distinct fork columns below its own
+#pragma unroll
+    for (int w = 0; w < PER; ++w) {
+      const int q = lane + 64 * w;
+      if (q < hs) {
+        const int st = st_sh[q];
+        bool first = st > 0;
+        for (int q3 = 0; q3 < q && first; ++q3) first = st_sh[q3] != st;
+        first_sh[q] = first ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    int slot_of[PER];
+    int reps = 0;
+#pragma unroll
+    for (int w = 0; w < PER; ++w) {
+      const int q = lane + 64 * w;
+      int slot = -1;
+      if (q < hs && st_sh[q] > 0) {
+        const int st = st_sh[q];
+        slot = 0;
+        for (int q2 = 0; q2 < hs; ++q2) slot += (first_sh[q2] && st_sh[q2] < st) ? 1 : 0;
+        if (first_sh[q]) {
+          ++reps;
+          if (slot < TRIE_SLOTS) p.trie_fork[h0 + slot] = st;
+        }
+      }
+      slot_of[w] = slot;
+    }
+#pragma unroll
+    for (int o = 32; o; o >>= 1) reps += __shfl_xor(reps, o);
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < PER; ++w) {
+      const int64_t q = lane + 64 * w;
+      if (q < hs && slot_of[w] >= TRIE_SLOTS) p.trie_start[h0 + q] = 0;
+    }
+    if (lane == 0) p.trie_nfork[k] = reps < TRIE_SLOTS ? reps : TRIE_SLOTS;
+    __syncthreads();
+  }
+}
+
 // unit -> bucket map by binary search over the scanned counts.
 __global__ void k_unit_map(const int64_t* __restrict__ scanned, int64_t nb, const unsigned long long* ctr,
                            int32_t* __restrict__ unit_bucket) {
@@ -161,25 +318,21 @@ __global__ void k_unit_map(const int64_t* __restrict__ scanned, int64_t nb, cons
   }
 }
 
-template <int R, bool ANCH, int U>
-__device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
-                                    uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ out) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-  const int tend = gp.m_max + 16;
-  const int tmain = gp.m_min + 2;
-  fill_codes(b, gp, codes, smax, tend + 18);
-  // row constants
-  float gd[R], bd[R], mu[R], be[R];
-  uint32_t rc[R];
-  const int np = gp.n + (ANCH ? 1 : 0);
+// Row constants of lane l's R rows for a read of n bases (bottom-aligned in
+// the group's 16R rows; rows above are pass-through rows).
+template <int R, bool ANCH>
+__device__ __forceinline__ void row_constants(const BatchDev& b, const QTab<float>* __restrict__ tab, int64_t read,
+                                              int n, int valid, int l, float (&gd)[R], float (&bd)[R],
+                                              float (&mu)[R], float (&be)[R], uint32_t (&rc)[R]) {
+  const int np = n + (ANCH ? 1 : 0);
   const int pad = GROUP * R - np;
-  const int64_t rb = b.read_off[gp.read];
+  const int64_t rb = b.read_off[read];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int rel = l * R + r - pad;
     float vg = 0.f, vb = 0.f, vm = 0.f, ve = 0.f;
     uint32_t vc = NOMATCH;
-    if (gp.valid) {
+    if (valid) {
       if (rel < 0) {
         vm = ve = 1.f;  // pass-through row
       } else if (ANCH && rel == 0) {
@@ -193,11 +346,23 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
         vb = tab->bad[q];
         vm = tab->mu[q];
         ve = tab->beta[q];
-        if (kk == gp.n - 1) vm = ve = tab->cfin[q];
+        if (kk == n - 1) vm = ve = tab->cfin[q];
       }
     }
     gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
   }
+}
+
+template <int R, bool ANCH, int U>
+__device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
+                                    uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+  const int tend = gp.m_max + 16;
+  const int tmain = gp.m_min + 2;
+  fill_codes(b, gp, codes, smax, tend + 18);
+  float gd[R], bd[R], mu[R], be[R];
+  uint32_t rc[R];
+  row_constants<R, ANCH>(b, tab, gp.read, gp.n, gp.valid, l, gd, bd, mu, be, rc);
   const float gam = tab->gamma, xx = tab->ext;
   const float gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : 0.f) : gam;
   __syncthreads();  // codes visible
@@ -287,6 +452,238 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
 // FP32 forward of the units of row class R (one kernel per class, so that each
 // runs with its own register footprint): the class's contiguous unit range in
 // the class-major plan, taken from a per-class counter.
+// ---------------------------------------------------------------------------
+// Prefix sharing (PlanDev::trie): a read unit gives group g read j*4+g of a
+// (region, class) bucket; the four groups sweep the region's haplotypes in
+// byte order in lock step (same haplotype, same loop bounds).  After step T of
+// a sweep, lane l holds column T-l-1 of its R rows, so the whole wavefront
+// after step f+1 (lane 0 at column f, every lane at a column <= f) depends on
+// the haplotype's first f bases only.  A sweep stores that wavefront -- each
+// lane's (M, D, Q) rows, its DPP pipeline values (Plast, Xprev) and its
+// terminal accumulator -- at every fork column f it computes; a haplotype
+// whose trie_start is s restores the wavefront of s and continues at step
+// s+2.  It then runs exactly the steps of a full sweep from there on, so the
+// results equal full sweeps bit for bit.
+
+struct ReadTask {
+  int64_t read, region;
+  int n, valid;
+  int q0, q1;  // ranks of the unit's haplotype chunk
+};
+
+__device__ __forceinline__ int decode_read_unit(const BatchDev& b, const PlanDev& p, unsigned long long u,
+                                                ReadTask& rt) {
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const int bkt = p.unit_bucket[u];
+  const int64_t j = (int64_t)u - p.unit_cnt[bkt];
+  const int64_t k = p.class_major ? bkt % b.nregions : bkt / NCLS;
+  const int c = p.class_major ? NCLS - 1 - (int)(bkt / b.nregions) : bkt % NCLS;
+  const int64_t r0 = b.region_reads[k];
+  const int32_t* co = p.class_off + k * (NCLS + 1);
+  const int64_t cnt = co[c + 1] - co[c];
+  const int64_t H = b.region_haps[k + 1] - b.region_haps[k];
+  const int64_t nch = (H + TRIE_CHUNK - 1) / TRIE_CHUNK;
+  const int64_t jr = j / nch, jc = j % nch;  // read group, haplotype chunk
+  const int64_t s = jr * PAIRS_PER_UNIT + g;
+  rt.valid = s < cnt;
+  rt.read = p.read_perm[r0 + co[c] + (rt.valid ? s : 0)];
+  rt.region = k;
+  rt.n = (int)(b.read_off[rt.read + 1] - b.read_off[rt.read]);
+  rt.q0 = (int)(jc * TRIE_CHUNK);
+  rt.q1 = (int)((jc + 1) * TRIE_CHUNK < H ? (jc + 1) * TRIE_CHUNK : H);
+  return c;
+}
+
+template <int R, bool ANCH, int U>
+__device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
+                                            uint16_t* codes, int smax, const ReadTask& rt,
+                                            double* __restrict__ out) {
+  constexpr int W = (3 * R + 3 + 3) & ~3;  // floats of one lane's wavefront state, padded to 16 bytes
+  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+  float gd[R], bd[R], mu[R], be[R];
+  uint32_t rc[R];
+  row_constants<R, ANCH>(b, tab, rt.read, rt.n, rt.valid, l, gd, bd, mu, be, rc);
+  const float gam = tab->gamma, xx = tab->ext;
+  const float gam_last = l == GROUP - 1 ? (ANCH ? tab->gap : 0.f) : gam;
+  const int64_t k = __builtin_amdgcn_readfirstlane((int)rt.region);  // one region per unit
+  const int64_t h0 = b.region_haps[k], H = b.region_haps[k + 1] - h0;
+  const int64_t r0 = b.region_reads[k];
+  const int nfork = p.trie_nfork[k];
+  // the region's fork columns, one per lane (sorted; lanes past nfork hold INT_MAX),
+  // so that per-haplotype lookups are ballots and lane reads, not memory chains
+  const int fork_v = lane < nfork ? p.trie_fork[h0 + lane] : 0x7fffffff;
+  auto fork_at = [&](int i) { return __builtin_amdgcn_readlane(fork_v, i); };
+  auto slot = [&](int f) {  // this lane's state in fork slot f
+    return p.trie_ws + ((((size_t)blockIdx.x * PAIRS_PER_UNIT + g) * TRIE_SLOTS + f) * GROUP + l) * W;
+  };
+  const float C = 0x1p108f;
+  const uint16_t* cl = codes + g * smax + 16 - l;
+  // this unit's chunk of ranks, one per lane (TRIE_CHUNK <= 64); the chunk's
+  // first haplotype sweeps in full, so every fork it restores was written
+  // inside the chunk
+  const int q0 = __builtin_amdgcn_readfirstlane(rt.q0), q1 = __builtin_amdgcn_readfirstlane(rt.q1);
+  int order_v = 0, start_v = 0, len_v = 0;
+  {
+    const int qq = q0 + lane;
+    order_v = qq < q1 ? p.trie_order[h0 + qq] : 0;
+    start_v = qq < q1 && lane > 0 ? p.trie_start[h0 + qq] : 0;
+    len_v = qq < q1 ? (int)(b.hap_off[order_v + 1] - b.hap_off[order_v]) : 0;
+  }
+  for (int qi = 0; qi < q1 - q0; ++qi) {
+    const int64_t hap = __builtin_amdgcn_readlane(order_v, qi);
+    const int m = __builtin_amdgcn_readlane(len_v, qi);
+    if (m == 0) {  // same haplotype for every group: uniform
+      if (rt.valid && l == 0) out[b.out_off[k] + (rt.read - r0) * H + (hap - h0)] = -__builtin_inf();
+      continue;
+    }
+    int st = __builtin_amdgcn_readlane(start_v, qi);
+    // fork slot of column st (a start without a slot sweeps in full), and the
+    // first fork this sweep computes: forks in (st, m] are saved after step f + 1
+    const unsigned long long hit = __builtin_amdgcn_ballot_w64(fork_v == st);
+    int rs = hit ? __builtin_ctzll(hit) : 0;
+    if (st > 0 && (!hit || st >= m)) st = 0;
+    int fi = __builtin_popcountll(__builtin_amdgcn_ballot_w64(fork_v <= st));
+    // a fork is saved only when a later haplotype of the chunk resumes from it
+    auto next_fork = [&]() {
+      while (fi < nfork && fork_at(fi) <= m &&
+             !__builtin_amdgcn_ballot_w64(start_v == fork_at(fi) && lane > qi && q0 + lane < q1))
+        ++fi;
+      return (fi < nfork && fork_at(fi) <= m) ? fork_at(fi) + 1 : 0x7fffffff;
+    };
+    int fstep = next_fork();
+    const int tend = m + 16, tmain = m + 2;
+    {
+      GroupPair hp;
+      hp.hap = hap;
+      hp.m = m;
+      hp.valid = 1;
+      fill_codes(b, hp, codes, smax, tend + 18);
+    }
+    __syncthreads();  // codes visible
+
+    float M[R], D[R], Q[R];
+    float Plast = 0.f, Xprev = 0.f, acc = 0.f, rmax = 0.f, latch = 0.f;
+    const float inj = ANCH ? 0.f : C;
+    auto main_step = [&](int tt) {
+      const uint32_t code = cl[tt];
+      float Pin = dpp_shr1(inj, Plast);
+      float Xn = dpp_shr1(inj, Q[R - 1]);
+      Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+      Xprev = Xn;
+      if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
+    };
+    auto tail_step = [&](int tt) {
+      const uint32_t code = cl[tt];
+      const float pinj = (!ANCH && tt - 1 > m) ? 0.f : inj;
+      float Pin = dpp_shr1(pinj, Plast);
+      float Xn = dpp_shr1(inj, Q[R - 1]);
+      Plast = adiag_step<float, R, true>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+      Xprev = Xn;
+      if (ANCH) {
+        rmax = fmaxf(rmax, Q[R - 1]);
+        if (tt - l - 1 == m) latch = Q[R - 1];
+      } else {
+        acc += Plast;
+      }
+    };
+    int t;
+    if (st == 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.f;
+      {  // step 0 (column -1 for lane 0): the boundary row's first values
+        const float inj0 = ANCH ? C : 0.f;
+        float Pin = dpp_shr1(inj0, Plast);
+        float Xn = dpp_shr1(inj0, Q[R - 1]);
+        Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[0]);
+        Xprev = Xn;
+        if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
+      }
+      t = 1;
+    } else {
+      float v[W];
+      const float4* src = reinterpret_cast<const float4*>(slot(rs));
+#pragma unroll
+      for (int i = 0; i < W / 4; ++i) {
+        const float4 x = src[i];
+        v[4 * i] = x.x;
+        v[4 * i + 1] = x.y;
+        v[4 * i + 2] = x.z;
+        v[4 * i + 3] = x.w;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        M[r] = v[3 * r];
+        D[r] = v[3 * r + 1];
+        Q[r] = v[3 * r + 2];
+      }
+      Plast = v[3 * R];
+      Xprev = v[3 * R + 1];
+      if (ANCH) rmax = v[3 * R + 2]; else acc = v[3 * R + 2];
+      t = st + 2;
+    }
+    // main steps in segments that end at the fork steps (fstep <= m + 1 < tmain),
+    // so that the unrolled loop itself carries no branch
+    for (;;) {
+      const int seg_end = fstep < tmain ? fstep : tmain - 1;  // last step of the segment
+      for (; t + U - 1 <= seg_end; t += U) {
+#pragma unroll
+        for (int kk = 0; kk < U; ++kk) main_step(t + kk);
+      }
+      for (; t <= seg_end; ++t) main_step(t);
+      if (seg_end != fstep) break;
+      if (rt.valid) {  // the wavefront after step f + 1, in 16-byte stores
+        float v[W];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          v[3 * r] = M[r];
+          v[3 * r + 1] = D[r];
+          v[3 * r + 2] = Q[r];
+        }
+        v[3 * R] = Plast;
+        v[3 * R + 1] = Xprev;
+        v[3 * R + 2] = ANCH ? rmax : acc;
+#pragma unroll
+        for (int i = 3 * R + 3; i < W; ++i) v[i] = 0.f;
+        float4* dst = reinterpret_cast<float4*>(slot(fi));
+#pragma unroll
+        for (int i = 0; i < W / 4; ++i) dst[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+      }
+      ++fi;
+      fstep = next_fork();
+    }
+    for (; t + 1 <= tend; t += 2) {
+      tail_step(t);
+      tail_step(t + 1);
+    }
+    if (t <= tend) tail_step(t++);
+    if (l == GROUP - 1 && rt.valid) {
+      const double lnk = (double)(p.read_exp[rt.read] - CEXP32) * LN2;
+      const int64_t oidx = b.out_off[k] + (rt.read - r0) * H + (hap - h0);
+      double v;
+      bool flag;
+      if (ANCH) {
+        flag = !(latch >= FLAG32_MIN) || !(rmax < __builtin_inff());
+        const double lc = log((double)rmax) + LN_1EM300;
+        const double ll = log((double)latch);
+        v = (ll > lc ? ll : lc) + lnk;
+      } else {
+        flag = !(acc >= FLAG32_MIN) || !(acc < __builtin_inff());
+        v = log((double)acc) + lnk - log((double)(m + 1));
+      }
+      if (flag) {
+        unsigned long long idx = atomicAdd(&p.ctr[CTR_F64], 1ull);
+        atomicAdd(&p.ctr[CTR_FLAGGED], 1ull);
+        if ((int64_t)idx < p.f64_cap) p.f64_list[idx] = PairRef{rt.read, hap, oidx};
+        else atomicAdd(&p.ctr[CTR_OVERFLOW], 1ull);
+        out[oidx] = __builtin_nan("");
+      } else {
+        out[oidx] = v;
+      }
+    }
+    __syncthreads();  // codes reused by the next haplotype
+  }
+}
+
 template <bool ANCH, int R, int U>
 __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                               int smax, double* __restrict__ out) {
@@ -311,6 +708,34 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
     GroupPair gp;
     if (decode_unit32(b, p, u, gp, out) != R) continue;
     unit32<R, ANCH, U>(b, p, tab, codes, smax, gp, out);
+  }
+}
+
+// The prefix-sharing forward of class R over the read units of the plan.
+template <bool ANCH, int R, int U>
+__global__ __launch_bounds__(64) void k_fwd32t(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
+                                               int smax, double* __restrict__ out) {
+  const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
+  const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
+  if (lo >= hi) return;
+  extern __shared__ __align__(16) unsigned char lds[];
+  QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds);
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
+    for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    unsigned long long u = 0;
+    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
+    u = __shfl(u, 0);
+    if (u >= hi) break;
+    ReadTask rt;
+    if (decode_read_unit(b, p, u, rt) != R) continue;
+    unit32_trie<R, ANCH, U>(b, p, tab, codes, smax, rt, out);
   }
 }
 
@@ -571,6 +996,37 @@ static void launch_fwd32_r(int r, int blocks, size_t lds, hipStream_t st, const 
   }
 }
 
+template <bool ANCH>
+static hipError_t occ_fwd32t(int r, int smax, int* n) {
+  switch (r) {
+#define VD_OCC(RR) \
+  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32t<ANCH, RR, 2>, 64, lds_fwd32(smax));
+    VD_OCC(1) VD_OCC(2) VD_OCC(3) VD_OCC(4) VD_OCC(5) VD_OCC(6) VD_OCC(7) VD_OCC(8)
+    VD_OCC(9) VD_OCC(10) VD_OCC(11) VD_OCC(12) VD_OCC(13) VD_OCC(14) VD_OCC(15) VD_OCC(16)
+#undef VD_OCC
+    default: *n = 1; return hipSuccess;
+  }
+}
+
+hipError_t occupancy_trie(int anchored, int r, int smax, int* blocks_per_cu) {
+  return anchored ? occ_fwd32t<true>(r, smax, blocks_per_cu) : occ_fwd32t<false>(r, smax, blocks_per_cu);
+}
+
+template <bool ANCH>
+static void launch_fwd32t_r(int r, int blocks, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
+                            const QTab<float>* tab32, int smax, double* out) {
+  switch (r) {
+#define VD_L(RR)                                                                                          \
+  case RR:                                                                                                \
+    hipLaunchKernelGGL((k_fwd32t<ANCH, RR, 2>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
+    break;
+    VD_L(1) VD_L(2) VD_L(3) VD_L(4) VD_L(5) VD_L(6) VD_L(7) VD_L(8)
+    VD_L(9) VD_L(10) VD_L(11) VD_L(12) VD_L(13) VD_L(14) VD_L(15) VD_L(16)
+#undef VD_L
+    default: break;
+  }
+}
+
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st) {
   const int64_t nb = b.nregions * NCLS;
@@ -583,6 +1039,7 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
   if (b.nregions > 0) {
     int g = (int)(b.nregions > 65535 ? 65535 : b.nregions);
     hipLaunchKernelGGL(k_bucket, dim3(g), dim3(256), 0, st, b, p, out);
+    if (p.trie) hipLaunchKernelGGL(k_hap_trie, dim3(g), dim3(64), 0, st, b, p);
   }
   hipLaunchKernelGGL(k_scan_units, dim3(1), dim3(1024), 0, st, p.unit_cnt, nb, p.ctr);
   hipLaunchKernelGGL(k_unit_map, dim3(2048), dim3(256), 0, st, (const int64_t*)p.unit_cnt, nb,
@@ -600,7 +1057,10 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
     for (int r = MAX_R32; r >= 1; --r) {  // largest class first
       if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
       const bool u4 = unroll32[r] == 4;
-      if (anchored) {
+      if (p.trie) {
+        if (anchored) launch_fwd32t_r<true>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+        else launch_fwd32t_r<false>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+      } else if (anchored) {
         if (u4) launch_fwd32_r<true, 4>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
         else launch_fwd32_r<true, 2>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
       } else {
