@@ -164,12 +164,19 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *   grad_variant        adjoint register block: 0 = 2 steps (two waves per SIMD
  *                       where it fits), 1 = 4 steps
  *   grad_blocks_per_cu  persistent adjoint grid per CU (0 = the occupancy limit)
- * Defaults: 2, 0, 0, 0.  Values never change results beyond rounding order. */
+ *   fwd_share           1: haplotype prefix sharing for the class (read units of
+ *                       share_chunk haplotypes), 0: pair units; applies while
+ *                       vd_prefix_sharing is on
+ *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
+ * Defaults: 2, 0, 0, 0, 1, 8.  Values never change results beyond rounding
+ * order (FP64-rescued pairs may differ in the last bit). */
 typedef struct vd_tuning {
   int fwd_unroll[17];
   int fwd_blocks_per_cu[17];
   int grad_variant[17];
   int grad_blocks_per_cu[17];
+  int fwd_share[17];
+  int share_chunk;
 } vd_tuning;
 
 int vd_tuning_get(vd_tuning *t);

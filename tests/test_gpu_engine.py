@@ -165,12 +165,18 @@ def test_autotune_keeps_results():
     rep2 = autotune.tune_grad(dev, mode=0, reps=1)
     assert rep["choice"] and rep2["choice"]
     for cand in autotune.FWD_CANDIDATES:
-        t = autotune.get()
-        for r in range(1, 17):
-            t.fwd_unroll[r], t.fwd_blocks_per_cu[r] = cand
-        autotune.set_tuning(t)
-        dev.forward(mode=0)
-        np.testing.assert_array_equal(dev.result(), ref)
+        for chunk in (1, 8, 64):
+            t = autotune.get()
+            t.share_chunk = chunk
+            for r in range(1, 17):
+                t.fwd_unroll[r], t.fwd_blocks_per_cu[r], t.fwd_share[r] = cand
+            autotune.set_tuning(t)
+            st = dev.forward(mode=0, stats=True)
+            got = dev.result()
+            # FP32 sweeps agree bit for bit; FP64-rescued pairs may move in the last bit
+            differ = np.flatnonzero(got != ref)
+            assert len(differ) <= st["fp64_pairs"]
+            np.testing.assert_allclose(got[differ], ref[differ], rtol=1e-13, atol=0)
     for cand in autotune.GRAD_CANDIDATES:
         t = autotune.get()
         for r in range(1, 17):

@@ -272,8 +272,21 @@ def test_tuning_get_set_validation():
     bad.grad_variant[7] = 2
     with pytest.raises(_lib.VdhmmError, match="grad_variant"):
         autotune.set_tuning(bad)
+    bad = _lib.VdTuning.from_buffer_copy(t)
+    bad.fwd_share[3] = 2
+    with pytest.raises(_lib.VdhmmError, match="fwd_share"):
+        autotune.set_tuning(bad)
+    for chunk in (0, 65):
+        bad = _lib.VdTuning.from_buffer_copy(t)
+        bad.share_chunk = chunk
+        with pytest.raises(_lib.VdhmmError, match="share_chunk"):
+            autotune.set_tuning(bad)
+    assert t.share_chunk == 8 and list(t.fwd_share)[1:] == [1] * 16   # defaults
     good = _lib.VdTuning.from_buffer_copy(t)
     good.fwd_unroll[10] = 4
+    good.fwd_share[10] = 0
+    good.share_chunk = 16
     autotune.set_tuning(good)
-    assert autotune.get().fwd_unroll[10] == 4
+    assert autotune.get().fwd_unroll[10] == 4 and autotune.get().fwd_share[10] == 0
+    assert autotune.get().share_chunk == 16
     autotune.set_tuning(t)

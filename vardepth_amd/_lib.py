@@ -58,10 +58,12 @@ class VdStats(ctypes.Structure):
 
 class VdTuning(ctypes.Structure):
     _fields_ = [("fwd_unroll", ctypes.c_int * 17), ("fwd_blocks_per_cu", ctypes.c_int * 17),
-                ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17)]
+                ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17),
+                ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int)]
 
     def as_dict(self):
-        return {name: list(getattr(self, name)) for name, _ in self._fields_}
+        return {name: (list(getattr(self, name)) if name != "share_chunk" else int(self.share_chunk))
+                for name, _ in self._fields_}
 
 
 class VdGenoBatch(ctypes.Structure):

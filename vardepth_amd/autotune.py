@@ -5,8 +5,10 @@ timed pass of a candidate setting gives every class's kernel time at once
 (per-class HIP events, ``vd_class_timing``); a sweep costs candidates x reps
 passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
 
-* forward: anti-diagonals per main-loop iteration (2 or 4) x persistent grid
-  per CU (the occupancy limit, or 2 or 3 blocks per CU);
+* forward: haplotype prefix sharing or pair units, anti-diagonals per
+  main-loop iteration (2 or 4, pair units) x persistent grid per CU (the
+  occupancy limit, or 2 or 3 blocks per CU); before the per-class sweep, the
+  haplotypes per read unit under sharing (4, 8 or 16) by total time;
 * adjoint: register block of 2 steps (two waves per SIMD where that fits) or
   4 steps.
 
@@ -25,7 +27,9 @@ import numpy as np
 
 from . import _lib
 
-FWD_CANDIDATES = [(2, 0), (4, 0), (2, 3), (4, 3), (2, 2)]   # (unroll, blocks per CU; 0 = occupancy limit)
+# (unroll, blocks per CU (0 = occupancy limit), prefix sharing); the shared kernel has one unroll
+FWD_CANDIDATES = [(2, 0, 1), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 0), (4, 3, 0), (2, 2, 1), (2, 2, 0)]
+CHUNK_CANDIDATES = (8, 4, 16)
 GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
 
 
@@ -79,7 +83,7 @@ def _pick(times, candidates, active):
 
 
 def _apply_fwd(t, r, cand):
-    t.fwd_unroll[r], t.fwd_blocks_per_cu[r] = cand
+    t.fwd_unroll[r], t.fwd_blocks_per_cu[r], t.fwd_share[r] = cand
 
 
 def _apply_grad(t, r, cand):
@@ -92,6 +96,21 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         dev_batch.forward(mode=mode, precision=precision, stats=True)
 
     run()  # warm: buffers and tables
+    chunk_ms = {}
+    base = get()
+    for chunk in CHUNK_CANDIDATES:  # whole-forward time of each read-unit size (sharing on everywhere)
+        t = _lib.VdTuning.from_buffer_copy(base)
+        t.share_chunk = chunk
+        for r in range(1, 17):
+            t.fwd_share[r] = 1
+        set_tuning(t)
+        best = np.inf
+        for _ in range(reps):
+            best = min(best, dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"])
+        chunk_ms[chunk] = best
+    t = _lib.VdTuning.from_buffer_copy(base)
+    t.share_chunk = min(chunk_ms, key=chunk_ms.get)
+    set_tuning(t)
     times = _sweep(run, candidates, _apply_fwd, 0, reps)
     default = times[candidates[0]]
     active = default > 0.02  # classes with work (an empty class kernel exits at once)
@@ -102,6 +121,7 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
     set_tuning(t)
     tuned = sum(times[c][r] for r, c in choice.items())
     return {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
+            "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
             "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
 
 
@@ -138,8 +158,13 @@ def load(path, key: str) -> bool:
     data = json.loads(p.read_text())
     if key not in data:
         return False
-    t = _lib.VdTuning()
+    t = get()
     for name, _ in t._fields_:
+        if name not in data[key]:
+            continue
+        if name == "share_chunk":
+            t.share_chunk = int(data[key][name])
+            continue
         arr = getattr(t, name)
         for i, v in enumerate(data[key][name]):
             arr[i] = int(v)

@@ -167,7 +167,9 @@ void tune_defaults() {
     g.tune.fwd_blocks_per_cu[r] = 0;
     g.tune.grad_variant[r] = 0;
     g.tune.grad_blocks_per_cu[r] = 0;
+    g.tune.fwd_share[r] = 1;
   }
+  g.tune.share_chunk = 8;
   g.tune_init = true;
 }
 
@@ -195,7 +197,7 @@ vd::PlanDev plan_dev(int64_t npairs, int class_major, int trie = 0) {
   return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
                      g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
                      g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
-                     class_major, trie, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
+                     class_major, trie, g.tune.share_chunk, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
                      g.trie_fork.as<int32_t>(), g.trie_nfork.as<int32_t>(), g.trie_ws.as<float>()};
 }
 
@@ -233,7 +235,9 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   }
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
-  const int trie = g.share && !force64 ? 1 : 0;
+  int trie = 0;  // class mask of the read units with prefix sharing
+  if (g.share && !force64)
+    for (int r = 1; r <= vd::MAX_R32; ++r) trie |= g.tune.fwd_share[r] ? (1 << r) : 0;
   if (trie) {
     if (g.occt_smax != smax) {
       for (int a = 0; a < 2; ++a)
@@ -242,6 +246,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     }
     size_t ws_floats = 0;
     for (int r = 1; r <= vd::MAX_R32; ++r) {
+      if (!((trie >> r) & 1)) continue;
       const int occ = std::max(1, g.occ32t[anchored][r]);
       const int want = g.tune.fwd_blocks_per_cu[r];
       blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
@@ -1050,7 +1055,9 @@ int vd_tuning_set(const vd_tuning* t) {
     if (t->fwd_unroll[r] != 2 && t->fwd_unroll[r] != 4) return fail("vdhmm: fwd_unroll must be 2 or 4");
     if (t->grad_variant[r] != 0 && t->grad_variant[r] != 1) return fail("vdhmm: grad_variant must be 0 or 1");
     if (t->fwd_blocks_per_cu[r] < 0 || t->grad_blocks_per_cu[r] < 0) return fail("vdhmm: negative grid size");
+    if (t->fwd_share[r] != 0 && t->fwd_share[r] != 1) return fail("vdhmm: fwd_share must be 0 or 1");
   }
+  if (t->share_chunk < 1 || t->share_chunk > 64) return fail("vdhmm: share_chunk must be in 1..64");
   std::lock_guard<std::mutex> lk(g_mu);
   g.tune = *t;
   g.tune_init = true;

@@ -95,7 +95,8 @@ struct PlanDev {
   // of one (region, class), each group sweeping the region's haplotypes in
   // lexicographic order and resuming each from the fork column it shares with
   // its predecessor
-  int trie;
+  int trie;                 // bit c set: class c runs read units with prefix sharing
+  int trie_chunk;           // haplotypes per read unit (1..64)
   int32_t* trie_order;      // [nhaps] region slot h0 + rank -> global haplotype index
   int32_t* trie_start;      // [nhaps] per rank: resume column s (0 = full sweep)
   int32_t* trie_fork;       // [nhaps] per region: sorted distinct fork columns at h0 + i, i < trie_nfork[k]
@@ -104,7 +105,6 @@ struct PlanDev {
 };
 
 constexpr int TRIE_SLOTS = 32;  // distinct fork columns kept per region (later forks are swept in full)
-constexpr int TRIE_CHUNK = 8;   // haplotypes per read unit: a region's sorted haplotypes split in chunks
 
 enum {
   CTR_UNIT32 = 0,   // next FP32 unit

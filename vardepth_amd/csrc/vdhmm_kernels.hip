@@ -112,12 +112,11 @@ __global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
       int c = threadIdx.x;
       // pair units (4 read x haplotype pairs), or with prefix sharing read
       // units (4 reads, each against every haplotype of the region)
-      const int64_t items = p.trie ? (int64_t)cnt[c] * ((H + TRIE_CHUNK - 1) / TRIE_CHUNK) : (int64_t)cnt[c] * H;
-      int64_t units = (c >= 1 && c <= MAX_R32 && H > 0)
-                          ? (p.trie ? ((int64_t)cnt[c] + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT *
-                                          ((H + TRIE_CHUNK - 1) / TRIE_CHUNK)
-                                    : (items + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT)
-                          : 0;
+      const bool tc = (p.trie >> c) & 1;
+      int64_t units = 0;
+      if (c >= 1 && c <= MAX_R32 && H > 0)
+        units = tc ? ((int64_t)cnt[c] + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT * ((H + p.trie_chunk - 1) / p.trie_chunk)
+                   : ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
       p.unit_cnt[bucket_of(c, k, b.nregions, p.class_major)] = units;
     }
     __syncthreads();
@@ -482,15 +481,15 @@ __device__ __forceinline__ int decode_read_unit(const BatchDev& b, const PlanDev
   const int32_t* co = p.class_off + k * (NCLS + 1);
   const int64_t cnt = co[c + 1] - co[c];
   const int64_t H = b.region_haps[k + 1] - b.region_haps[k];
-  const int64_t nch = (H + TRIE_CHUNK - 1) / TRIE_CHUNK;
+  const int64_t nch = (H + p.trie_chunk - 1) / p.trie_chunk;
   const int64_t jr = j / nch, jc = j % nch;  // read group, haplotype chunk
   const int64_t s = jr * PAIRS_PER_UNIT + g;
   rt.valid = s < cnt;
   rt.read = p.read_perm[r0 + co[c] + (rt.valid ? s : 0)];
   rt.region = k;
   rt.n = (int)(b.read_off[rt.read + 1] - b.read_off[rt.read]);
-  rt.q0 = (int)(jc * TRIE_CHUNK);
-  rt.q1 = (int)((jc + 1) * TRIE_CHUNK < H ? (jc + 1) * TRIE_CHUNK : H);
+  rt.q0 = (int)(jc * p.trie_chunk);
+  rt.q1 = (int)((jc + 1) * p.trie_chunk < H ? (jc + 1) * p.trie_chunk : H);
   return c;
 }
 
@@ -518,7 +517,7 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
   };
   const float C = 0x1p108f;
   const uint16_t* cl = codes + g * smax + 16 - l;
-  // this unit's chunk of ranks, one per lane (TRIE_CHUNK <= 64); the chunk's
+  // this unit's chunk of ranks, one per lane (trie_chunk <= 64); the chunk's
   // first haplotype sweeps in full, so every fork it restores was written
   // inside the chunk
   const int q0 = __builtin_amdgcn_readfirstlane(rt.q0), q1 = __builtin_amdgcn_readfirstlane(rt.q1);
@@ -1057,7 +1056,7 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
     for (int r = MAX_R32; r >= 1; --r) {  // largest class first
       if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
       const bool u4 = unroll32[r] == 4;
-      if (p.trie) {
+      if ((p.trie >> r) & 1) {
         if (anchored) launch_fwd32t_r<true>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
         else launch_fwd32t_r<false>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
       } else if (anchored) {
