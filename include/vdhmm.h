@@ -117,6 +117,8 @@ typedef struct vd_stats {
   double fwd32_ms;       /* GPU time of the FP32 anti-diagonal kernel */
   double fwd64_ms;       /* GPU time of the FP64 kernel (rescue / long / forced) */
   double total_ms;       /* wall time of the call */
+  uint64_t fp32_cells_swept;  /* read rows x haplotype columns the FP32 kernels computed
+                                 (below the pairs' n*m when haplotype prefixes are shared) */
 } vd_stats;
 
 /* Reference error table 10^(-min(q,60)/10) for the 64 bins, and gap / ext. */
@@ -168,7 +170,7 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *                       share_chunk haplotypes), 0: pair units; applies while
  *                       vd_prefix_sharing is on
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
- * Defaults: 2, 0, 0, 0, 1, 8.  Values never change results beyond rounding
+ * Defaults: 2, 0, 0, 0, 1, 4.  Values never change results beyond rounding
  * order (FP64-rescued pairs may differ in the last bit). */
 typedef struct vd_tuning {
   int fwd_unroll[17];

@@ -50,7 +50,8 @@ class VdStats(ctypes.Structure):
     _fields_ = [("pairs", ctypes.c_uint64), ("cells", ctypes.c_uint64), ("fp32_units", ctypes.c_uint64),
                 ("flagged", ctypes.c_uint64), ("fp64_pairs", ctypes.c_uint64), ("fp64_bad", ctypes.c_uint64),
                 ("overflow", ctypes.c_uint64), ("device_ms", ctypes.c_double), ("plan_ms", ctypes.c_double),
-                ("fwd32_ms", ctypes.c_double), ("fwd64_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("fwd32_ms", ctypes.c_double), ("fwd64_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("fp32_cells_swept", ctypes.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -29,7 +29,7 @@ from . import _lib
 
 # (unroll, blocks per CU (0 = occupancy limit), prefix sharing); the shared kernel has one unroll
 FWD_CANDIDATES = [(2, 0, 1), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 0), (4, 3, 0), (2, 2, 1), (2, 2, 0)]
-CHUNK_CANDIDATES = (8, 4, 16)
+CHUNK_CANDIDATES = (4, 8, 16)
 GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
 
 

@@ -56,6 +56,16 @@ __device__ __forceinline__ T adiag_step(T (&M)[R], T (&D)[R], T (&Q)[R], const T
   return Ih;
 }
 
+// Add v (the value of lane 16g of each group g, 0 for the others) to a 64-bit
+// counter: one atomic per wave.
+__device__ __forceinline__ void count_groups(unsigned long long* ctr, unsigned long long v) {
+  const int lane = threadIdx.x & 63;
+  v = (lane & 15) ? 0ull : v;
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  if (lane == 0 && v) atomicAdd(ctr, v);
+}
+
 // Per-lane description of the lane's group pair inside a unit.
 struct GroupPair {
   int64_t read, hap, out;

@@ -169,7 +169,7 @@ void tune_defaults() {
     g.tune.grad_blocks_per_cu[r] = 0;
     g.tune.fwd_share[r] = 1;
   }
-  g.tune.share_chunk = 8;
+  g.tune.share_chunk = 4;
   g.tune_init = true;
 }
 
@@ -293,6 +293,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     stats->fp64_bad = c[vd::CTR_F64BAD];
     stats->overflow = c[vd::CTR_OVERFLOW];
     stats->device_ms = ms;
+    stats->fp32_cells_swept = force64 ? 0 : c[vd::CTR_SWEPT];
     if (c[vd::CTR_OVERFLOW]) return fail("vdhmm: FP64 work list overflow");
   }
   return 0;
@@ -489,6 +490,7 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
     stats->fp64_bad = c[vd::CTR_F64BAD];
     stats->overflow = c[vd::CTR_OVERFLOW];
     stats->device_ms = ms;
+    stats->fp32_cells_swept = c[vd::CTR_SWEPT];
     if (c[vd::CTR_OVERFLOW]) return fail("vdhmm: FP64 work list overflow");
   }
   return 0;

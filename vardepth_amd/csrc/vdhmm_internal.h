@@ -114,6 +114,7 @@ enum {
   CTR_OVERFLOW = 4, // FP64 list overflow (pairs dropped: reported as error)
   CTR_NUNITS = 5,   // total FP32 units (written by the scan)
   CTR_F64BAD = 6,   // FP64 results that were not finite and positive
+  CTR_SWEPT = 7,    // FP32 cells swept (read rows x haplotype columns computed)
   CTR_CLS = 8,      // [8 + c]: next unit of class c (per-class gradient kernels)
   CTR_BIN64 = 32,   // [32 + b]: FP64 gradient sub-list lengths (8 row bins)
   CTR_COUNT = 40

@@ -421,6 +421,7 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
     tail_step(t + 1);
   }
   if (t <= tend) tail_step(t++);
+  count_groups(&p.ctr[CTR_SWEPT], gp.valid ? (unsigned long long)gp.n * (unsigned long long)gp.m : 0ull);
   if (l == GROUP - 1 && gp.valid) {
     const double lnk = (double)(p.read_exp[gp.read] - CEXP32) * LN2;
     double v;
@@ -655,6 +656,7 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
       tail_step(t + 1);
     }
     if (t <= tend) tail_step(t++);
+    count_groups(&p.ctr[CTR_SWEPT], rt.valid ? (unsigned long long)rt.n * (unsigned long long)(m - st) : 0ull);
     if (l == GROUP - 1 && rt.valid) {
       const double lnk = (double)(p.read_exp[rt.read] - CEXP32) * LN2;
       const int64_t oidx = b.out_off[k] + (rt.read - r0) * H + (hap - h0);
