@@ -280,8 +280,9 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "autotune": {"forward": tune_f, "adjoint": tune_g}}
 
 
-def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float, threads: int):
-    """Forward + diploid genotype likelihood on the chromosome-scale sample."""
+def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float, threads: int, tune: bool = True):
+    """Forward + diploid genotype likelihood on the chromosome-scale sample
+    (launch choices tuned on the sample itself unless tune is False)."""
     import torch
     import oracle
     from vardepth_amd import engine, workloads
@@ -297,6 +298,10 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
         gl_off = engine.gl_offsets(nh)
         dev = engine.DeviceBatch(b)
         mq = torch.from_numpy(cs.mq).cuda()
+        tuning = None
+        if tune:
+            from vardepth_amd import autotune
+            tuning = autotune.tune_forward(dev, mode=mode)
         dev.forward(mode=mode)
         dev.genotype(mq)
         torch.cuda.synchronize()
@@ -357,7 +362,8 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
             "regions": b.nregions, "pairs": b.npairs, "genotype_outputs": int(gl_off[-1]), "cells": cells,
             "generate_s": round(gen_s, 1),
             "end_to_end_ms": round(wall * 1e3, 3), "end_to_end_gcups": round(cells / wall / 1e9, 2),
-            "forward_ms": round(fst["device_ms"], 3), "molecule_ms": round(gst["molecule_ms"], 3),
+            "forward_ms": round(fst["device_ms"], 3), "fp32_cells_swept": int(fst["fp32_cells_swept"]),
+            "autotune": tuning, "molecule_ms": round(gst["molecule_ms"], 3),
             "mixture_ms": round(gst["mixture_ms"], 3), "mixture_units": gst["units"],
             "kept_rows": gst["kept_rows"], "invalid_regions": gst["invalid"],
             "mixture_roofline": {"bound": "valu-fp64", "terms": terms, "flops_per_term": 4,
@@ -423,7 +429,7 @@ def main():
             del src
         print(json.dumps({"genotype": genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")],
                                                      args.mode, max(1, min(args.steps, 3)), args.geno_cpu_seconds,
-                                                     min(available_cpus(), 16))}), flush=True)
+                                                     min(available_cpus(), 16), not args.no_tune)}), flush=True)
         return
 
     if args.no_forward:
@@ -503,7 +509,8 @@ def main():
     genotype = None
     if args.geno_mb > 0:
         genotype = genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")], args.mode,
-                                  max(1, min(args.steps, 3)), args.geno_cpu_seconds, min(available_cpus(), 16))
+                                  max(1, min(args.steps, 3)), args.geno_cpu_seconds, min(available_cpus(), 16),
+                                  not args.no_tune)
 
     if args.copy_calib:
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")

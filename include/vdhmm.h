@@ -166,12 +166,16 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *   grad_variant        adjoint register block: 0 = 2 steps (two waves per SIMD
  *                       where it fits), 1 = 4 steps
  *   grad_blocks_per_cu  persistent adjoint grid per CU (0 = the occupancy limit)
- *   fwd_share           1: haplotype prefix sharing for the class (read units of
- *                       share_chunk haplotypes), 0: pair units; applies while
- *                       vd_prefix_sharing is on
+ *   fwd_share           0: pair units; 1: haplotype prefix sharing for the class
+ *                       (read units of share_chunk haplotypes); 2: prefix sharing
+ *                       and the shared-suffix continuation; 3: 2 where the
+ *                       suffix kernel keeps the occupancy of the prefix-only
+ *                       kernel, else 1.  Applies while vd_prefix_sharing is on
+ *                       (the suffix part while vd_suffix_sharing is on)
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
- * Defaults: 2, 0, 0, 0, 1, 4.  Values never change results beyond rounding
- * order (FP64-rescued pairs may differ in the last bit). */
+ * Defaults: 2, 0, 0, 0, 3, 4.  Values never change results beyond rounding
+ * order (FP64-rescued pairs may differ in the last bit; the suffix
+ * continuation agrees with full sweeps to FP32 rounding). */
 typedef struct vd_tuning {
   int fwd_unroll[17];
   int fwd_blocks_per_cu[17];
@@ -191,6 +195,16 @@ int vd_tuning_set(const vd_tuning *t);
  * on < 0 only queries; returns the previous setting.  The FP64 path and the
  * gradients always sweep in full. */
 int vd_prefix_sharing(int on);
+/* Shared-suffix continuation of the prefix-sharing forward (default on;
+ * applies while vd_prefix_sharing is on; the reference's CPU trie_many does
+ * the same reduction, trie_vector.cpp:205-237,250-253): when every haplotype
+ * of a region ends in the same >= 16 bases, each read runs one adjoint
+ * (backward) sweep over them, and every haplotype is swept only up to the
+ * suffix, its terminal completed by a dot product with the adjoint.  Results
+ * equal full sweeps to FP32 rounding (the adjoint and the FP64 dot product
+ * reorder the sums).  on = 1 / 0 sets it, on < 0 only queries; returns the
+ * previous setting. */
+int vd_suffix_sharing(int on);
 /* When on, calls made with a stats pointer also time each class kernel;
  * vd_class_ms returns the last such times (17 each, index = class). */
 void vd_class_timing(int on);

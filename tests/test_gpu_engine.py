@@ -153,8 +153,10 @@ def test_plain_baseline_matches():
 
 
 def test_autotune_keeps_results():
-    """Every launch choice gives the same likelihoods and gradients."""
+    """Every launch choice gives the same likelihoods and gradients (with the
+    suffix continuation off: it agrees to FP32 rounding, tested on its own)."""
     from vardepth_amd import autotune
+    prev_sfx = _lib.lib().vd_suffix_sharing(0)
     b = workloads.uneven_regions(seed=9, regions=24)
     dev = engine.DeviceBatch(b)
     base = autotune.get()
@@ -185,6 +187,7 @@ def test_autotune_keeps_results():
         g = dev.grad(mode=0).cpu().numpy()
         np.testing.assert_allclose(g, g_ref, rtol=1e-12, atol=1e-12)
     autotune.set_tuning(base)
+    _lib.lib().vd_suffix_sharing(prev_sfx)
 
 
 def _sharing_cases():
@@ -213,6 +216,7 @@ def _sharing_cases():
 def test_prefix_sharing_matches_full_sweeps_bit_for_bit(mode):
     L = _lib.lib()
     prev = L.vd_prefix_sharing(-1)
+    prev_sfx = L.vd_suffix_sharing(0)
     try:
         for b in _sharing_cases():
             L.vd_prefix_sharing(0)
@@ -228,3 +232,73 @@ def test_prefix_sharing_matches_full_sweeps_bit_for_bit(mode):
             np.testing.assert_allclose(shared[differ], full[differ], rtol=1e-13, atol=0)
     finally:
         L.vd_prefix_sharing(prev)
+        L.vd_suffix_sharing(prev_sfx)
+
+
+def _suffix_cases():
+    rng = np.random.default_rng(91)
+    seq = lambda n: "".join(rng.choice(list("ACGT"), n))
+    tail = seq(120)
+    left = seq(200)
+    mut = lambda s, i, c: s[:i] + c + s[i + 1:]
+    other = lambda s, i: mut(s, i, "A" if s[i] != "A" else "C")
+    hap0 = left + tail
+    reads = [hap0[i:i + L] for i, L in ((0, 60), (150, 150), (200, 120), (250, 70), (300, 20), (90, 255))]
+    reads.append(seq(100))  # unrelated read: tiny likelihoods (rescued or clamped)
+    quals = [bytes(rng.integers(5, 41, len(r)).astype(np.uint8)) for r in reads]
+    edge = [
+        # suffix of 120 after haplotypes that differ near the start, in the middle and just before it
+        (reads, quals, [hap0, other(hap0, 3), other(hap0, 150), other(hap0, 199), left[:80] + tail]),
+        # one haplotype is the suffix itself (it sweeps column 0 only)
+        (reads[:3], quals[:3], [hap0, tail, other(hap0, 10)]),
+        # identical haplotypes (the whole haplotype is the suffix)
+        (reads[:4], quals[:4], [hap0] * 4),
+        # common suffix of exactly 16 and of 15 (below the minimum: no continuation)
+        (reads[:3], quals[:3], [seq(100) + tail[-16:], seq(90) + tail[-16:], seq(120) + tail[-16:]]),
+        (reads[:3], quals[:3], [seq(100) + tail[-15:], "A" + seq(89) + tail[-15:]]),
+        # many haplotypes sharing the suffix, beyond one read unit and beyond 32 forks
+        (reads[:2], quals[:2], [other(hap0, 5 * i) for i in range(40)]),
+    ]
+    cases = [workloads.chromosome_sample(mb=0.05, depth=30).batch, workloads.uneven_regions(seed=8, regions=48),
+             workloads.from_regions(edge)]
+    return cases
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_suffix_continuation_matches_full_sweeps(mode):
+    """The shared-suffix continuation (one adjoint sweep per read over the
+    suffix every haplotype of a region ends in, then a dot product with each
+    haplotype's state at m - S) agrees with full sweeps to FP32 rounding, and
+    sweeps fewer cells where a suffix is shared."""
+    from vardepth_amd import autotune
+    L = _lib.lib()
+    prev = L.vd_suffix_sharing(-1)
+    base = autotune.get()
+    try:
+        t = autotune.get()
+        for chunk in (4, 2):
+            t.share_chunk = chunk
+            for r in range(1, 17):
+                t.fwd_share[r] = 2
+            autotune.set_tuning(t)
+            for i, b in enumerate(_suffix_cases()):
+                L.vd_suffix_sharing(0)
+                full, st0 = engine.forward(b, mode=mode, stats=True)
+                L.vd_suffix_sharing(1)
+                sfx, st1 = engine.forward(b, mode=mode, stats=True)
+                assert st1["fp32_cells_swept"] <= st0["fp32_cells_swept"]
+                if i != 1:  # the chromosome sample and the edge cases share suffixes
+                    assert st1["fp32_cells_swept"] < st0["fp32_cells_swept"], (i, st0, st1)
+                both = np.isfinite(full) & np.isfinite(sfx)
+                assert np.array_equal(np.isfinite(full), np.isfinite(sfx))
+                assert np.array_equal(np.isnan(full), np.isnan(sfx))
+                np.testing.assert_allclose(sfx[both], full[both], rtol=0, atol=3e-5)
+                # and against the oracle (the reference model in FP64)
+                if i == 2:
+                    expect = np.concatenate([
+                        [oracle.dense(r, q, h, mode, b.gap, b.ext) for r, q in zip(*b.region(k)[:2])
+                         for h in b.region(k)[2]] for k in range(b.nregions)])
+                    close(sfx, expect, TOL_MIXED)
+    finally:
+        L.vd_suffix_sharing(prev)
+        autotune.set_tuning(base)

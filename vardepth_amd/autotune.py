@@ -5,14 +5,17 @@ timed pass of a candidate setting gives every class's kernel time at once
 (per-class HIP events, ``vd_class_timing``); a sweep costs candidates x reps
 passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
 
-* forward: haplotype prefix sharing or pair units, anti-diagonals per
-  main-loop iteration (2 or 4, pair units) x persistent grid per CU (the
-  occupancy limit, or 2 or 3 blocks per CU); before the per-class sweep, the
-  haplotypes per read unit under sharing (4, 8 or 16) by total time;
+* forward: pair units, haplotype prefix sharing, or prefix sharing with the
+  shared-suffix continuation; anti-diagonals per main-loop iteration (2 or 4,
+  pair units) x persistent grid per CU (the occupancy limit, or 2 or 3 blocks
+  per CU); before the per-class sweep, the haplotypes per read unit under
+  sharing (4, 8 or 16) by total time;
 * adjoint: register block of 2 steps (two waves per SIMD where that fits) or
   4 steps.
 
-None of them changes the arithmetic of a pair, so results are identical for
+None of them changes the arithmetic of a pair except the suffix
+continuation (which sums the suffix columns through an adjoint and agrees
+with full sweeps to FP32 rounding), so results are otherwise identical for
 every choice; only the time differs.  ``tune_forward`` / ``tune_grad`` apply
 the per-class best and return a report; ``save`` / ``load`` keep a choice
 in a JSON file keyed by device and mode.
@@ -27,8 +30,10 @@ import numpy as np
 
 from . import _lib
 
-# (unroll, blocks per CU (0 = occupancy limit), prefix sharing); the shared kernel has one unroll
-FWD_CANDIDATES = [(2, 0, 1), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 0), (4, 3, 0), (2, 2, 1), (2, 2, 0)]
+# (unroll, blocks per CU (0 = occupancy limit), sharing: 0 pair units, 1 prefix, 2 prefix +
+# suffix, 3 suffix where it keeps the occupancy); the shared kernels have one unroll
+FWD_CANDIDATES = [(2, 0, 3), (2, 0, 1), (2, 0, 2), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 2), (2, 3, 0), (4, 3, 0),
+                  (2, 2, 1), (2, 2, 2), (2, 2, 0)]
 CHUNK_CANDIDATES = (4, 8, 16)
 GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
 
@@ -102,7 +107,7 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         t = _lib.VdTuning.from_buffer_copy(base)
         t.share_chunk = chunk
         for r in range(1, 17):
-            t.fwd_share[r] = 1
+            t.fwd_share[r] = 3
         set_tuning(t)
         best = np.inf
         for _ in range(reps):

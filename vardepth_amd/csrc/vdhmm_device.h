@@ -26,6 +26,21 @@ __device__ __forceinline__ double dpp_shr1(double old, double src) {
   return __builtin_bit_cast(double, r);
 }
 
+// row_shl:1, the reverse hand-off (lane l reads lane l+1; `old` enters lane 15)
+__device__ __forceinline__ float dpp_shl1(float old, float src) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
+                                         0x101, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ double dpp_shl1(double old, double src) {
+  I2 o = __builtin_bit_cast(I2, old), s = __builtin_bit_cast(I2, src);
+  I2 r;
+  r.lo = __builtin_amdgcn_update_dpp(o.lo, s.lo, 0x101, 0xF, 0xF, false);
+  r.hi = __builtin_amdgcn_update_dpp(o.hi, s.hi, 0x101, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -54,6 +69,26 @@ __device__ __forceinline__ T adiag_step(T (&M)[R], T (&D)[R], T (&Q)[R], const T
     Ih = P;
   }
   return Ih;
+}
+
+// x summed over the 16 lanes of each group (every lane gets the sum), by
+// ds_swizzle xor butterflies: the lane pattern is an immediate, so no lane
+// index registers stay live around the call.
+template <int XM>
+__device__ __forceinline__ double swz_xor(double v) {
+  I2 a = __builtin_bit_cast(I2, v);
+  constexpr int pat = (XM << 10) | 0x1f;  // bitmask mode: and 0x1f, or 0, xor XM
+  a.lo = __builtin_amdgcn_ds_swizzle(a.lo, pat);
+  a.hi = __builtin_amdgcn_ds_swizzle(a.hi, pat);
+  return __builtin_bit_cast(double, a);
+}
+
+__device__ __forceinline__ double group_sum(double v) {
+  v += swz_xor<1>(v);
+  v += swz_xor<2>(v);
+  v += swz_xor<4>(v);
+  v += swz_xor<8>(v);
+  return v;
 }
 
 // Add v (the value of lane 16g of each group g, 0 for the others) to a 64-bit

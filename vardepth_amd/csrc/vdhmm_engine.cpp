@@ -83,9 +83,10 @@ struct Engine {
   int gocc_smax = -1;
   // haplotype prefix sharing of the FP32 forward
   bool share = true;
-  int occ32t[2][17] = {};
+  bool suffix = true;
+  int occ32t[2][17][2] = {};  // [anchored][class][suffix continuation]
   int occt_smax = -1;
-  DevBuf trie_order, trie_start, trie_fork, trie_nfork, trie_ws;
+  DevBuf trie_order, trie_start, trie_fork, trie_nfork, trie_ws, trie_sfx, sfx_ws;
 };
 
 Engine g;
@@ -167,7 +168,7 @@ void tune_defaults() {
     g.tune.fwd_blocks_per_cu[r] = 0;
     g.tune.grad_variant[r] = 0;
     g.tune.grad_blocks_per_cu[r] = 0;
-    g.tune.fwd_share[r] = 1;
+    g.tune.fwd_share[r] = 3;
   }
   g.tune.share_chunk = 4;
   g.tune_init = true;
@@ -198,7 +199,8 @@ vd::PlanDev plan_dev(int64_t npairs, int class_major, int trie = 0) {
                      g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
                      g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
                      class_major, trie, g.tune.share_chunk, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
-                     g.trie_fork.as<int32_t>(), g.trie_nfork.as<int32_t>(), g.trie_ws.as<float>()};
+                     g.trie_fork.as<int32_t>(), g.trie_nfork.as<int32_t>(), g.trie_ws.as<float>(),
+                     0, g.trie_sfx.as<int32_t>(), g.sfx_ws.as<float>()};
 }
 
 int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
@@ -238,20 +240,31 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   int trie = 0;  // class mask of the read units with prefix sharing
   if (g.share && !force64)
     for (int r = 1; r <= vd::MAX_R32; ++r) trie |= g.tune.fwd_share[r] ? (1 << r) : 0;
+  int sfx = 0;  // class mask of the read units with the suffix continuation
   if (trie) {
     if (g.occt_smax != smax) {
       for (int a = 0; a < 2; ++a)
-        for (int r = 1; r <= vd::MAX_R32; ++r) HIPCHK(vd::occupancy_trie(a, r, smax, &g.occ32t[a][r]));
+        for (int r = 1; r <= vd::MAX_R32; ++r)
+          for (int x = 0; x < 2; ++x) HIPCHK(vd::occupancy_trie(a, r, x, smax, &g.occ32t[a][r][x]));
       g.occt_smax = smax;
     }
-    size_t ws_floats = 0;
+    // fwd_share 2: always; 3 (default): where the suffix kernel keeps the
+    // occupancy of the prefix-only kernel (its code costs registers)
+    if (g.suffix)
+      for (int r = 1; r <= vd::MAX_R32; ++r) {
+        const int fs = g.tune.fwd_share[r];
+        const bool keep = g.occ32t[anchored][r][1] >= g.occ32t[anchored][r][0];
+        sfx |= (fs == 2 || (fs == 3 && keep)) ? (1 << r) : 0;
+      }
+    size_t ws_floats = 0, sfx_floats = 0;
     for (int r = 1; r <= vd::MAX_R32; ++r) {
       if (!((trie >> r) & 1)) continue;
-      const int occ = std::max(1, g.occ32t[anchored][r]);
+      const int occ = std::max(1, g.occ32t[anchored][r][(sfx >> r) & 1]);
       const int want = g.tune.fwd_blocks_per_cu[r];
       blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
       ws_floats = std::max(ws_floats, (size_t)blocks32[r] * vd::PAIRS_PER_UNIT * vd::TRIE_SLOTS * vd::GROUP *
                                           ((3 * r + 3 + 3) & ~3));
+      sfx_floats = std::max(sfx_floats, (size_t)blocks32[r] * vd::SFX_WS);
     }
     const int64_t nh = std::max<int64_t>(db->nhaps, 1), nreg = std::max<int64_t>(db->nregions, 1);
     HIPCHK(g.trie_order.ensure(nh * 4));
@@ -259,9 +272,12 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     HIPCHK(g.trie_fork.ensure(nh * 4));
     HIPCHK(g.trie_nfork.ensure(nreg * 4));
     HIPCHK(g.trie_ws.ensure(ws_floats * sizeof(float)));
+    HIPCHK(g.trie_sfx.ensure(nreg * 4));
+    if (sfx) HIPCHK(g.sfx_ws.ensure(sfx_floats * sizeof(float)));
   }
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs, 1, trie);
+  vd::PlanDev p = plan_dev(npairs, 1, trie);
+  p.sfx = sfx;
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, unroll32, blocks64,
@@ -1057,7 +1073,7 @@ int vd_tuning_set(const vd_tuning* t) {
     if (t->fwd_unroll[r] != 2 && t->fwd_unroll[r] != 4) return fail("vdhmm: fwd_unroll must be 2 or 4");
     if (t->grad_variant[r] != 0 && t->grad_variant[r] != 1) return fail("vdhmm: grad_variant must be 0 or 1");
     if (t->fwd_blocks_per_cu[r] < 0 || t->grad_blocks_per_cu[r] < 0) return fail("vdhmm: negative grid size");
-    if (t->fwd_share[r] != 0 && t->fwd_share[r] != 1) return fail("vdhmm: fwd_share must be 0 or 1");
+    if (t->fwd_share[r] < 0 || t->fwd_share[r] > 3) return fail("vdhmm: fwd_share must be 0, 1, 2 or 3");
   }
   if (t->share_chunk < 1 || t->share_chunk > 64) return fail("vdhmm: share_chunk must be in 1..64");
   std::lock_guard<std::mutex> lk(g_mu);
@@ -1070,6 +1086,13 @@ int vd_prefix_sharing(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   const int was = g.share ? 1 : 0;
   if (on >= 0) g.share = on != 0;
+  return was;
+}
+
+int vd_suffix_sharing(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int was = g.suffix ? 1 : 0;
+  if (on >= 0) g.suffix = on != 0;
   return was;
 }
 

@@ -53,20 +53,6 @@ constexpr int NSUB = 2;      // register blocks per checkpoint
 constexpr int CKW_MAX = 2 * MAX_R32 + 2;
 constexpr int NBIN64 = 8;    // FP64 gradient row bins: R = 2, 4, ..., 16
 
-__device__ __forceinline__ float dpp_shl1(float old, float src) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old), __builtin_bit_cast(int, src),
-                                         0x101, 0xF, 0xF, false));
-}
-
-__device__ __forceinline__ double dpp_shl1(double old, double src) {
-  I2 o = __builtin_bit_cast(I2, old), s = __builtin_bit_cast(I2, src);
-  I2 r;
-  r.lo = __builtin_amdgcn_update_dpp(o.lo, s.lo, 0x101, 0xF, 0xF, false);
-  r.hi = __builtin_amdgcn_update_dpp(o.hi, s.hi, 0x101, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, r);
-}
-
 template <typename T>
 struct GradScale;
 template <>

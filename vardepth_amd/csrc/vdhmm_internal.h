@@ -102,9 +102,19 @@ struct PlanDev {
   int32_t* trie_fork;       // [nhaps] per region: sorted distinct fork columns at h0 + i, i < trie_nfork[k]
   int32_t* trie_nfork;      // [nregions]
   float* trie_ws;           // fork wavefronts: [block][group][TRIE_SLOTS][16 lanes][3R+3 padded to 4]
+  // suffix continuation (sfx = 1): a region whose haplotypes all end in the
+  // same S >= 16 bases gets, per read, one adjoint sweep over those S columns;
+  // each haplotype is then swept to column m - S only and its terminal
+  // completed by a dot product with the adjoint
+  int sfx;                  // bit c set: class c runs with the suffix continuation (planner finds S if any)
+  int32_t* trie_sfx;        // [nregions] common suffix S (0: none)
+  float* sfx_ws;            // [block][64 lanes][84]: adjoint weights, boundary mass, the column m - S state
 };
 
 constexpr int TRIE_SLOTS = 32;  // distinct fork columns kept per region (later forks are swept in full)
+constexpr int SFX_MIN = 16;     // shortest common suffix worth the adjoint sweep
+constexpr int SFX_SEED_EXP = 100;  // adjoint seed 2^100 (FP32 headroom on both sides)
+constexpr int SFX_WS = (5 * MAX_R32 + 4) * 64;  // floats of sfx_ws per block (64 lanes x 84)
 
 enum {
   CTR_UNIT32 = 0,   // next FP32 unit
@@ -156,7 +166,7 @@ namespace vd {
 size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
-hipError_t occupancy_trie(int anchored, int r, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
+hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,

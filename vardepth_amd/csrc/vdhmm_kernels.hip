@@ -179,6 +179,22 @@ __device__ __forceinline__ int hap_cmp_wave(const BatchDev& b, int64_t i, int64_
   return (int)n;
 }
 
+// Common suffix length of haplotypes i and j, compared from the ends 64 bytes
+// per step by the whole wave.  All lanes must call it with the same arguments.
+__device__ __forceinline__ int hap_sfx_wave(const BatchDev& b, int64_t i, int64_t j) {
+  const int lane = threadIdx.x & 63;
+  const int64_t ei = b.hap_off[i + 1], ej = b.hap_off[j + 1];
+  const int64_t li = ei - b.hap_off[i], lj = ej - b.hap_off[j];
+  const int64_t n = li < lj ? li : lj;
+  for (int64_t t0 = 0; t0 < n; t0 += 64) {
+    const int64_t t = t0 + lane;
+    const bool diff = t < n && b.hap_bytes[ei - 1 - t] != b.hap_bytes[ej - 1 - t];
+    const unsigned long long mask = __builtin_amdgcn_ballot_w64(diff);
+    if (mask) return (int)(t0 + __builtin_ctzll(mask));
+  }
+  return (int)n;
+}
+
 // One wave (block) per region: haplotypes in byte order (trie_order, by binary
 // insertion: O(H log H) wave-wide compares, each starting past the prefix all
 // of the region's haplotypes share), the column each resumes from
@@ -190,6 +206,13 @@ __device__ __forceinline__ int hap_cmp_wave(const BatchDev& b, int64_t i, int64_
 // computed it, and every haplotype in between resumed past j, so by the chain
 // of common prefixes it shares the reader's first j bases); byte order makes
 // neighbours share the most.
+//
+// With the suffix continuation on, the suffix S every haplotype of the region
+// ends in (trie_sfx; 0 below SFX_MIN or with one haplotype) is not swept per
+// haplotype: each haplotype h runs to column J_h = m_h - S.  Its start is then
+// also at most J_h - 1 and at most J_{h-1} - 1 of its predecessor, so that
+// along any chain of resumes the haplotype that last computed a fork column f
+// computes it before its own column J (f + 1 <= J).
 __global__ __launch_bounds__(64) void k_hap_trie(BatchDev b, PlanDev p) {
   constexpr int PER = (TRIE_MAXH + 63) / 64;  // ranks per lane
   __shared__ int sorted_sh[TRIE_MAXH];
@@ -206,6 +229,16 @@ __global__ __launch_bounds__(64) void k_hap_trie(BatchDev b, PlanDev p) {
         common = c < common ? c : common;
       }
       if (H <= 1) common = 0;
+      int sfx = 0;
+      if (p.sfx && H > 1) {
+        sfx = 0x7fffffff;
+        for (int64_t i = 1; i < H; ++i) {
+          const int c = hap_sfx_wave(b, h0, h0 + i);
+          sfx = c < sfx ? c : sfx;
+        }
+        if (sfx < SFX_MIN) sfx = 0;
+      }
+      if (lane == 0) p.trie_sfx[k] = sfx;
       // binary insertion into sorted_sh[0..i)
       for (int i = 0; i < (int)H; ++i) {
         int lo = 0, hi = i;
@@ -238,7 +271,10 @@ __global__ __launch_bounds__(64) void k_hap_trie(BatchDev b, PlanDev p) {
       }
     } else {
       for (int64_t i = lane; i < H; i += 64) p.trie_order[h0 + i] = (int32_t)(h0 + i);
+      if (lane == 0) p.trie_sfx[k] = 0;
     }
+    __syncthreads();
+    const int S = p.trie_sfx[k];
     __syncthreads();
     __shared__ int st_sh[TRIE_MAXH];
     __shared__ int first_sh[TRIE_MAXH];
@@ -246,10 +282,12 @@ __global__ __launch_bounds__(64) void k_hap_trie(BatchDev b, PlanDev p) {
       int st = 0;
       if (share && q > 0) {
         const int64_t a = p.trie_order[h0 + q - 1], c = p.trie_order[h0 + q];
-        const int m = (int)(b.hap_off[c + 1] - b.hap_off[c]);
+        const int m = (int)(b.hap_off[c + 1] - b.hap_off[c]) - S;
+        const int ma = (int)(b.hap_off[a + 1] - b.hap_off[a]) - S;
         bool ib;
         st = hap_cmp_wave(b, a, c, 0, ib);
         st = st < m - 1 ? st : m - 1;
+        if (S) st = st < ma - 1 ? st : ma - 1;
         st = st > 0 ? st : 0;
       }
       if (lane == 0) {
@@ -494,7 +532,83 @@ __device__ __forceinline__ int decode_read_unit(const BatchDev& b, const PlanDev
   return c;
 }
 
+// ---------------------------------------------------------------------------
+// Suffix continuation (PlanDev::sfx).  When every haplotype of a region ends
+// in the same S bases, the terminal of a pair is linear in the sweep's state
+// at column J = m - S: the columns after J see that state only through each
+// row's Q_i(J) (the diagonal source of M_{i+1}(J+1)) and D^_i(J+1) =
+// M^_i(J) + x D^_i(J), plus the boundary injections at columns > J.  The
+// weights are the adjoints of the recurrence over the suffix columns,
+// which depend on the read and the suffix only:
+//   Qb_i(j) = e'_{i+1}(j+1) Mb_{i+1}(j+1)
+//   Db_i(j) = gamma_i Qb_i(j) + x Db_i(j+1)
+//   Mb_i(j) = Pb_i(j) + Qb_i(j) + Db_i(j+1)
+//   Pb_{i-1}(j) = mu_i Pb_i(j) + beta_i Qb_i(j)
+// seeded with Pb_n(j) = 2^SFX_SEED_EXP at every suffix column (semiglobal
+// terminal sum) or Qb_n(m) = 2^SFX_SEED_EXP (anchored terminal).  The sweep
+// runs anti-diagonals backwards: lane l is at suffix column
+// k = S + 15 - l - t after step t (k = 1 is column J + 1) and takes the row
+// below it from lane l + 1 through DPP row_shl:1.  Per row it leaves
+// Qb_i(J) and Db_i(J+1) in the unit's workspace; lane 0 adds up the adjoints
+// of the injected boundary values (dT/dP_in at columns J+1..m, dT/dQ_in at
+// columns J..m-1), which the haplotype sweeps add at scale 2^CEXP32.
+// The reference's CPU trie does the same reduction over a shared suffix
+// (trie_vector.cpp:205-237, 250-253) in FP64 with its own backward arrays.
 template <int R, bool ANCH, int U>
+__device__ __forceinline__ float sfx_adjoint(const float (&gd)[R], const float (&bd)[R], const float (&mu)[R],
+                                             const float (&be)[R], const uint32_t (&rc)[R], float gam,
+                                             float gam_last, float xx, const uint16_t* codes, int S,
+                                             float2* __restrict__ coef) {
+  const int lane = threadIdx.x & 63, l = lane & 15;
+  float Db[R], Qc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) Db[r] = Qc[r] = 0.f;
+  const float seed = 0x1p100f;
+  static_assert(SFX_SEED_EXP == 100, "seed constant");
+  if (ANCH && l == GROUP - 1) Qc[R - 1] = seed;
+  const float injP = ANCH ? 0.f : seed;
+  float PaOut = 0.f, YOut = 0.f, F = 0.f;
+  const uint16_t* cr = codes + (S + 32 - l);  // cr[-t]: code of this lane's column at step t
+  auto astep = [&](int tt, bool acc) {
+    const uint32_t code = cr[-tt];
+    float Pa = dpp_shl1(injP, PaOut);
+    float pend = dpp_shl1(0.f, YOut);
+#pragma unroll
+    for (int r = R - 1; r >= 0; --r) {
+      const float Qb = Qc[r];
+      Qc[r] = pend;
+      const float Mb = Pa + Qb + Db[r];
+      Db[r] = fmaT(r == R - 1 ? gam_last : gam, Qb, xx * Db[r]);
+      Pa = fmaT(mu[r], Pa, be[r] * Qb);
+      pend = (rc[r] == code ? gd[r] : bd[r]) * Mb;
+    }
+    if (!ANCH && acc) F += Pa + pend;
+    PaOut = Pa;
+    YOut = pend;
+  };
+  // steps 0..14: lane 0 above the suffix; 15..S-2: every lane inside; the last
+  // 16 steps: lane l reaches column J + 1 at step S + 14 - l and stores
+  int t = 0;
+#pragma unroll 1
+  for (; t < 15; ++t) astep(t, false);
+  for (; t + U - 1 < S - 1; t += U) {
+#pragma unroll
+    for (int kk = 0; kk < U; ++kk) astep(t + kk, true);
+  }
+#pragma unroll 1
+  for (; t < S - 1; ++t) astep(t, true);
+#pragma unroll 1
+  for (; t < S + 15; ++t) {
+    astep(t, true);
+    if (t == S + 14 - l) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) coef[r] = make_float2(Qc[r], Db[r]);
+    }
+  }
+  return F;
+}
+
+template <int R, bool ANCH, int U, bool SFX>
 __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p, const QTab<float>* __restrict__ tab,
                                             uint16_t* codes, int smax, const ReadTask& rt,
                                             double* __restrict__ out) {
@@ -529,6 +643,38 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
     start_v = qq < q1 && lane > 0 ? p.trie_start[h0 + qq] : 0;
     len_v = qq < q1 ? (int)(b.hap_off[order_v + 1] - b.hap_off[order_v]) : 0;
   }
+  // suffix continuation: one adjoint sweep per read when the chunk holds two
+  // or more haplotypes (S >= SFX_MIN guarantees that columns J+1..J+15 exist)
+  const int S = (SFX && q1 - q0 >= 2) ? __builtin_amdgcn_readfirstlane(p.trie_sfx[k]) : 0;
+  // this lane's workspace (lane-major, so that every access is an immediate
+  // offset from one address): per row (Qb_i(J), Db_i(J+1)); the boundary
+  // mass; then Q_i(J), M^_i(J) and D^_i(J) of the current haplotype
+  float2* const sbase = reinterpret_cast<float2*>(p.sfx_ws + (size_t)blockIdx.x * SFX_WS);  // uniform
+  const int slane = lane * (SFX_WS / 128);  // this lane's first float2
+  float* const sstate = reinterpret_cast<float*>(sbase + slane + MAX_R32 + 2);
+  if (S > 0) {
+    {  // suffix codes of the first haplotype: slot k + 17 holds suffix column k
+      const int64_t hb = b.hap_off[order_v], hend = b.hap_off[order_v + 1];
+      const int64_t hs = __builtin_amdgcn_readfirstlane((int)(hend - hb)) - S;
+      const int64_t base = __builtin_amdgcn_readfirstlane(hb) + hs;
+      for (int kk = lane; kk < S + 34; kk += 64) {
+        const int col = kk - 17;
+        codes[kk] = (uint16_t)(col <= 0 ? SENT_PRE : (col > S ? SENT_POST : (uint32_t)b.hap_bytes[base + col - 1]));
+      }
+    }
+    __syncthreads();
+    const float F = sfx_adjoint<R, ANCH, U>(gd, bd, mu, be, rc, gam, gam_last, xx, codes, S, sbase + slane);
+    // the group's lane 0 holds the boundary mass: every lane keeps its group's
+    // value in its own slot (lane 15 forms the terminal)
+    const int fb = __builtin_bit_cast(int, F);
+    const float f0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(fb, 0));
+    const float f1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(fb, 16));
+    const float f2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(fb, 32));
+    const float f3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(fb, 48));
+    sbase[slane + MAX_R32] = make_float2(g == 0 ? f0 : g == 1 ? f1 : g == 2 ? f2 : f3, 0.f);
+    count_groups(&p.ctr[CTR_SWEPT], rt.valid ? (unsigned long long)rt.n * (unsigned long long)S : 0ull);
+    __syncthreads();  // codes reused by the haplotypes
+  }
   for (int qi = 0; qi < q1 - q0; ++qi) {
     const int64_t hap = __builtin_amdgcn_readlane(order_v, qi);
     const int m = __builtin_amdgcn_readlane(len_v, qi);
@@ -537,21 +683,25 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
       continue;
     }
     int st = __builtin_amdgcn_readlane(start_v, qi);
+    // the sweep's last column: J = m - S with the suffix continuation, else m
+    const int J = m - S;
     // fork slot of column st (a start without a slot sweeps in full), and the
-    // first fork this sweep computes: forks in (st, m] are saved after step f + 1
+    // first fork this sweep computes: forks in (st, flast] are saved after step
+    // f + 1, inside the main loop
+    const int flast = S ? J - 1 : m;
     const unsigned long long hit = __builtin_amdgcn_ballot_w64(fork_v == st);
     int rs = hit ? __builtin_ctzll(hit) : 0;
-    if (st > 0 && (!hit || st >= m)) st = 0;
+    if (st > 0 && (!hit || st > flast || st >= m)) st = 0;
     int fi = __builtin_popcountll(__builtin_amdgcn_ballot_w64(fork_v <= st));
     // a fork is saved only when a later haplotype of the chunk resumes from it
     auto next_fork = [&]() {
-      while (fi < nfork && fork_at(fi) <= m &&
+      while (fi < nfork && fork_at(fi) <= flast &&
              !__builtin_amdgcn_ballot_w64(start_v == fork_at(fi) && lane > qi && q0 + lane < q1))
         ++fi;
-      return (fi < nfork && fork_at(fi) <= m) ? fork_at(fi) + 1 : 0x7fffffff;
+      return (fi < nfork && fork_at(fi) <= flast) ? fork_at(fi) + 1 : 0x7fffffff;
     };
     int fstep = next_fork();
-    const int tend = m + 16, tmain = m + 2;
+    const int tend = J + 16, tmain = S ? J + 1 : m + 2;
     {
       GroupPair hp;
       hp.hap = hap;
@@ -651,22 +801,73 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
       ++fi;
       fstep = next_fork();
     }
+    // the tail, steps up to tend = J + 16 (lane 15 at column J last).  With the
+    // suffix continuation the tail is steps J+1..J+16: columns <= J+15 < m, so
+    // no sentinel codes and no latch; lane l is at column J after step
+    // J + l + 1 and leaves Q_i(J) and D^_i(J+1) in the workspace
+    auto tail_snap = [&](int tt) {
+      tail_step(tt);
+      if (SFX && S && tt - l - 1 == J) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          sstate[r] = Q[r];
+          sstate[MAX_R32 + r] = M[r];
+          sstate[2 * MAX_R32 + r] = D[r];
+        }
+      }
+    };
     for (; t + 1 <= tend; t += 2) {
-      tail_step(t);
-      tail_step(t + 1);
+      tail_snap(t);
+      tail_snap(t + 1);
     }
-    if (t <= tend) tail_step(t++);
-    count_groups(&p.ctr[CTR_SWEPT], rt.valid ? (unsigned long long)rt.n * (unsigned long long)(m - st) : 0ull);
+    if (t <= tend) tail_snap(t++);
+    double dot = 0.0;
+    if (SFX && S) {
+      // terminal of the suffix columns: the adjoint weights times that state.
+      // The weights do not change between haplotypes; the empty asm keeps the
+      // compiler from holding them in registers across the sweeps
+      int so = slane;
+      asm volatile("" : "+v"(so));
+      // two rows at a time: the row constants of the next sweep stay live
+      // here, so the loads are kept from all being issued at once
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float2 w = sbase[so + r];
+        const float* x = reinterpret_cast<const float*>(sbase + so + MAX_R32 + 2);
+        const float dn = fmaT(xx, x[2 * MAX_R32 + r], x[MAX_R32 + r]);  // D^_i(J+1), as the sweep forms it
+        dot += (double)w.x * (double)x[r] + (double)w.y * (double)dn;
+        if (r & 1) asm volatile("" ::: "memory");
+      }
+      dot = group_sum(dot) * 0x1p-100;
+    }
+    count_groups(&p.ctr[CTR_SWEPT], rt.valid ? (unsigned long long)rt.n * (unsigned long long)(J - st) : 0ull);
     if (l == GROUP - 1 && rt.valid) {
       const double lnk = (double)(p.read_exp[rt.read] - CEXP32) * LN2;
       const int64_t oidx = b.out_off[k] + (rt.read - r0) * H + (hap - h0);
       double v;
       bool flag;
-      if (ANCH) {
+      if (ANCH && SFX) {
+        // with the suffix continuation rmax covers columns <= J only; it bounds
+        // nothing there: an unflagged latch >= 2^-100 is within 2^-228 of any
+        // finite FP32 row value, far above the reference's 1e-300 clamp
+        const double lt = S ? dot : (double)latch;
+        flag = !(lt >= FLAG32_MIN) || !(rmax < __builtin_inff()) || !(lt < 0x1p128);
+        const double lc = log((double)rmax) + LN_1EM300;
+        const double ll = log(lt);
+        v = (ll > lc ? ll : lc) + lnk;
+      } else if (ANCH) {
         flag = !(latch >= FLAG32_MIN) || !(rmax < __builtin_inff());
         const double lc = log((double)rmax) + LN_1EM300;
         const double ll = log((double)latch);
         v = (ll > lc ? ll : lc) + lnk;
+      } else if (SFX) {
+        // the boundary injections of the suffix columns carry 2^CEXP32
+        int so = slane;
+        asm volatile("" : "+v"(so));
+        const double tot = S ? (double)acc + dot + (double)sbase[so + MAX_R32].x * 0x1p8 : (double)acc;
+        static_assert(CEXP32 - SFX_SEED_EXP == 8, "injection scale");
+        flag = !(tot >= FLAG32_MIN) || !(tot < 0x1p128);
+        v = log(tot) + lnk - log((double)(m + 1));
       } else {
         flag = !(acc >= FLAG32_MIN) || !(acc < __builtin_inff());
         v = log((double)acc) + lnk - log((double)(m + 1));
@@ -712,8 +913,10 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
   }
 }
 
-// The prefix-sharing forward of class R over the read units of the plan.
-template <bool ANCH, int R, int U>
+// The prefix-sharing forward of class R over the read units of the plan (SFX:
+// with the suffix continuation, a separate instantiation because its code
+// raises the register count of some classes).
+template <bool ANCH, int R, int U, bool SFX>
 __global__ __launch_bounds__(64) void k_fwd32t(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                                int smax, double* __restrict__ out) {
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
@@ -736,7 +939,7 @@ __global__ __launch_bounds__(64) void k_fwd32t(BatchDev b, PlanDev p, const QTab
     if (u >= hi) break;
     ReadTask rt;
     if (decode_read_unit(b, p, u, rt) != R) continue;
-    unit32_trie<R, ANCH, U>(b, p, tab, codes, smax, rt, out);
+    unit32_trie<R, ANCH, U, SFX>(b, p, tab, codes, smax, rt, out);
   }
 }
 
@@ -997,11 +1200,11 @@ static void launch_fwd32_r(int r, int blocks, size_t lds, hipStream_t st, const 
   }
 }
 
-template <bool ANCH>
+template <bool ANCH, bool SFX>
 static hipError_t occ_fwd32t(int r, int smax, int* n) {
   switch (r) {
 #define VD_OCC(RR) \
-  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32t<ANCH, RR, 2>, 64, lds_fwd32(smax));
+  case RR: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_fwd32t<ANCH, RR, 2, SFX>, 64, lds_fwd32(smax));
     VD_OCC(1) VD_OCC(2) VD_OCC(3) VD_OCC(4) VD_OCC(5) VD_OCC(6) VD_OCC(7) VD_OCC(8)
     VD_OCC(9) VD_OCC(10) VD_OCC(11) VD_OCC(12) VD_OCC(13) VD_OCC(14) VD_OCC(15) VD_OCC(16)
 #undef VD_OCC
@@ -1009,17 +1212,18 @@ static hipError_t occ_fwd32t(int r, int smax, int* n) {
   }
 }
 
-hipError_t occupancy_trie(int anchored, int r, int smax, int* blocks_per_cu) {
-  return anchored ? occ_fwd32t<true>(r, smax, blocks_per_cu) : occ_fwd32t<false>(r, smax, blocks_per_cu);
+hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_per_cu) {
+  if (sfx) return anchored ? occ_fwd32t<true, true>(r, smax, blocks_per_cu) : occ_fwd32t<false, true>(r, smax, blocks_per_cu);
+  return anchored ? occ_fwd32t<true, false>(r, smax, blocks_per_cu) : occ_fwd32t<false, false>(r, smax, blocks_per_cu);
 }
 
-template <bool ANCH>
+template <bool ANCH, bool SFX>
 static void launch_fwd32t_r(int r, int blocks, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
                             const QTab<float>* tab32, int smax, double* out) {
   switch (r) {
 #define VD_L(RR)                                                                                          \
   case RR:                                                                                                \
-    hipLaunchKernelGGL((k_fwd32t<ANCH, RR, 2>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
+    hipLaunchKernelGGL((k_fwd32t<ANCH, RR, 2, SFX>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
     break;
     VD_L(1) VD_L(2) VD_L(3) VD_L(4) VD_L(5) VD_L(6) VD_L(7) VD_L(8)
     VD_L(9) VD_L(10) VD_L(11) VD_L(12) VD_L(13) VD_L(14) VD_L(15) VD_L(16)
@@ -1059,8 +1263,14 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
       if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
       const bool u4 = unroll32[r] == 4;
       if ((p.trie >> r) & 1) {
-        if (anchored) launch_fwd32t_r<true>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
-        else launch_fwd32t_r<false>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
+        const size_t lds = lds_fwd32(smax);
+        if ((p.sfx >> r) & 1) {
+          if (anchored) launch_fwd32t_r<true, true>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
+          else launch_fwd32t_r<false, true>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
+        } else {
+          if (anchored) launch_fwd32t_r<true, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
+          else launch_fwd32t_r<false, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
+        }
       } else if (anchored) {
         if (u4) launch_fwd32_r<true, 4>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
         else launch_fwd32_r<true, 2>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
