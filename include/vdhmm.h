@@ -189,7 +189,7 @@ int vd_tuning_get(vd_tuning *t);
 int vd_tuning_set(const vd_tuning *t);
 /* Haplotype prefix sharing of the FP32 forward (default on; no counterpart
  * call in the reference, whose CPU trie_many shares prefixes the same way,
- * trie_vector.cpp:198-204): each read sweeps its region's haplotypes in byte
+ * trie_vector.cpp:204-210,259-273): each read sweeps its region's haplotypes in byte
  * order and resumes every haplotype from the column it shares with the
  * previous one, with results identical to full sweeps.  on = 1 / 0 sets it,
  * on < 0 only queries; returns the previous setting.  The FP64 path and the
@@ -197,7 +197,7 @@ int vd_tuning_set(const vd_tuning *t);
 int vd_prefix_sharing(int on);
 /* Shared-suffix continuation of the prefix-sharing forward (default on;
  * applies while vd_prefix_sharing is on; the reference's CPU trie_many does
- * the same reduction, trie_vector.cpp:205-237,250-253): when every haplotype
+ * the same reduction, trie_vector.cpp:230-257,274-275): when every haplotype
  * of a region ends in the same >= 16 bases, each read runs one adjoint
  * (backward) sweep over them, and every haplotype is swept only up to the
  * suffix, its terminal completed by a dot product with the adjoint.  Results

@@ -553,7 +553,7 @@ __device__ __forceinline__ int decode_read_unit(const BatchDev& b, const PlanDev
 // of the injected boundary values (dT/dP_in at columns J+1..m, dT/dQ_in at
 // columns J..m-1), which the haplotype sweeps add at scale 2^CEXP32.
 // The reference's CPU trie does the same reduction over a shared suffix
-// (trie_vector.cpp:205-237, 250-253) in FP64 with its own backward arrays.
+// (trie_vector.cpp:230-257, 274-275) in FP64 with its own backward arrays.
 template <int R, bool ANCH, int U>
 __device__ __forceinline__ float sfx_adjoint(const float (&gd)[R], const float (&bd)[R], const float (&mu)[R],
                                              const float (&be)[R], const uint32_t (&rc)[R], float gam,

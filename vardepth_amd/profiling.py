@@ -16,6 +16,10 @@ runs, so FETCH_SIZE / WRITE_SIZE can be scaled to bytes per dispatch.
     python -m vardepth_amd.profiling traffic FETCH_DB WRITE_DB [CALIB_DB]
         HBM bytes per dispatch per kernel, each counter scaled by what it
         reads for the 256 MiB copy (taken from CALIB_DB when given)
+    python -m vardepth_amd.profiling passes KT_DB N
+        the last N passes of the batch (each starts at a k_prep_reads
+        dispatch): per kernel, dispatches, average and total duration, i.e.
+        the timed passes after an autotuning sweep, without its candidates
     python -m vardepth_amd.profiling families KT_DB [FETCH_DB WRITE_DB]
         kernels grouped by family (the template instances of one kernel, e.g.
         the per-class k_fwd32<A, R, U>): variants, dispatches per variant, the
@@ -130,7 +134,25 @@ def family_table(kt_db: str, fetch_db: str | None = None, write_db: str | None =
     return sorted(fam.items(), key=lambda kv: -kv[1]["total"])
 
 
+def last_passes(db: str, n: int):
+    """[(name, dispatches, avg_ms, total_ms)] of the kernels dispatched from the
+    start of the n-th last k_prep_reads on, sorted by total time."""
+    con = sqlite3.connect(db)
+    starts = [r[0] for r in con.execute("select start from kernels where name like '%k_prep_reads%' "
+                                        "order by start desc limit ?", (int(n),)).fetchall()]
+    if not starts:
+        return []
+    rows = con.execute("select name, count(*), avg(duration), sum(duration) from kernels where start >= ? "
+                       "group by name order by sum(duration) desc", (min(starts),)).fetchall()
+    return [(r[0], r[1], r[2] / 1e6, r[3] / 1e6) for r in rows]
+
+
 def main(argv):
+    if len(argv) >= 3 and argv[0] == "passes":
+        print("%-72s %6s %10s %10s" % ("kernel", "calls", "avg ms", "total ms"))
+        for r in last_passes(argv[1], int(argv[2])):
+            print("%-72s %6d %10.3f %10.3f" % (r[0][:72], r[1], r[2], r[3]))
+        return 0
     if len(argv) >= 2 and argv[0] == "families":
         rows = family_table(*argv[1:4])
         print("%-22s %8s %7s %14s %12s %14s %14s" % ("family", "variants", "calls", "sum avg ms", "total ms",
