@@ -886,8 +886,25 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
   }
 }
 
+// Waves per SIMD requested for the class kernels whose natural register
+// allocation stops just short of the next occupancy step (e.g. 171 registers
+// where 168 give 3 waves): the highest occupancy at which the compiler fits
+// the instance without spilling, found by compiling every instance at 2..7
+// waves.  1 = no request (the hint would not raise the occupancy, and it
+// changes the schedule).  Index: rows per lane R (0 unused).
+constexpr int WPE_T[2][2][17] = {{{1, 1, 6, 1, 1, 1, 1, 1, 1, 4, 1, 1, 1, 3, 3, 1, 1},
+                                  {1, 1, 1, 5, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
+                                 {{1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+                                  {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}}};  // [anchored][suffix][R]
+constexpr int WPE_P[2][2][17] = {{{1, 8, 8, 1, 6, 1, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1},
+                                  {1, 8, 8, 7, 6, 5, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1}},
+                                 {{1, 1, 1, 7, 6, 1, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1},
+                                  {1, 1, 1, 7, 6, 5, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1}}};  // [anchored][unroll 4][R]
+constexpr int wpe_t(bool anch, bool sfx, int r) { return WPE_T[anch][sfx][r]; }
+constexpr int wpe_p(bool anch, int u, int r) { return WPE_P[anch][u == 4][r]; }
+
 template <bool ANCH, int R, int U>
-__global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_p(ANCH, U, R)))) void k_fwd32(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                               int smax, double* __restrict__ out) {
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
   const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
@@ -917,7 +934,7 @@ __global__ __launch_bounds__(64) void k_fwd32(BatchDev b, PlanDev p, const QTab<
 // with the suffix continuation, a separate instantiation because its code
 // raises the register count of some classes).
 template <bool ANCH, int R, int U, bool SFX>
-__global__ __launch_bounds__(64) void k_fwd32t(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, SFX, R)))) void k_fwd32t(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
                                                int smax, double* __restrict__ out) {
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
   const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
