@@ -235,7 +235,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     const int want = g.tune.fwd_blocks_per_cu[r];
     blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
   }
-  const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 8)));
+  const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 16)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   int trie = 0;  // class mask of the read units with prefix sharing
   if (g.share && !force64)
