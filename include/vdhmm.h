@@ -173,7 +173,7 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *                       kernel, else 1.  Applies while vd_prefix_sharing is on
  *                       (the suffix part while vd_suffix_sharing is on)
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
- * Defaults: 2, 0, 0, 0, 3, 4.  Values never change results beyond rounding
+ * Defaults: 2, 0, 0, 0, 2, 4.  Values never change results beyond rounding
  * order (FP64-rescued pairs may differ in the last bit; the suffix
  * continuation agrees with full sweeps to FP32 rounding). */
 typedef struct vd_tuning {

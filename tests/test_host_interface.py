@@ -281,7 +281,7 @@ def test_tuning_get_set_validation():
         bad.share_chunk = chunk
         with pytest.raises(_lib.VdhmmError, match="share_chunk"):
             autotune.set_tuning(bad)
-    assert t.share_chunk == 4 and list(t.fwd_share)[1:] == [3] * 16   # defaults
+    assert t.share_chunk == 4 and list(t.fwd_share)[1:] == [2] * 16   # defaults
     good = _lib.VdTuning.from_buffer_copy(t)
     good.fwd_unroll[10] = 4
     good.fwd_share[10] = 0

@@ -31,8 +31,9 @@ import numpy as np
 from . import _lib
 
 # (unroll, blocks per CU (0 = occupancy limit), sharing: 0 pair units, 1 prefix, 2 prefix +
-# suffix, 3 suffix where it keeps the occupancy); the shared kernels have one unroll
-FWD_CANDIDATES = [(2, 0, 3), (2, 0, 1), (2, 0, 2), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 2), (2, 3, 0), (4, 3, 0),
+# suffix, 3 suffix where it keeps the occupancy); the shared kernels have one unroll.  The
+# first is the default.
+FWD_CANDIDATES = [(2, 0, 2), (2, 0, 1), (2, 0, 3), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 2), (2, 3, 0), (4, 3, 0),
                   (2, 2, 1), (2, 2, 2), (2, 2, 0)]
 CHUNK_CANDIDATES = (4, 8, 16)
 GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
@@ -107,7 +108,7 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         t = _lib.VdTuning.from_buffer_copy(base)
         t.share_chunk = chunk
         for r in range(1, 17):
-            t.fwd_share[r] = 3
+            t.fwd_share[r] = 2
         set_tuning(t)
         best = np.inf
         for _ in range(reps):

@@ -91,14 +91,30 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-// Add v (the value of lane 16g of each group g, 0 for the others) to a 64-bit
-// counter: one atomic per wave.
+// The value of lane 16g (g = 0..3), read into scalar registers.  Cross-group
+// exchanges use readlane with constant lanes rather than shuffles: shuffle
+// lane addresses are loop-invariant VGPRs the compiler keeps live across the
+// persistent loops.
+__device__ __forceinline__ int lane16(int v, int g) { return __builtin_amdgcn_readlane(v, 16 * g); }
+__device__ __forceinline__ int64_t lane16(int64_t v, int g) {
+  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)v, 16 * g);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), 16 * g);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ unsigned long long first_lane(unsigned long long v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Add v (the value of lane 16g of each group g) to a 64-bit counter: one
+// atomic per wave.
 __device__ __forceinline__ void count_groups(unsigned long long* ctr, unsigned long long v) {
   const int lane = threadIdx.x & 63;
-  v = (lane & 15) ? 0ull : v;
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  if (lane == 0 && v) atomicAdd(ctr, v);
+  const int64_t w = (int64_t)v;
+  const unsigned long long t =
+      (unsigned long long)(lane16(w, 0) + lane16(w, 1) + lane16(w, 2) + lane16(w, 3));
+  if (lane == 0 && t) atomicAdd(ctr, t);
 }
 
 // Per-lane description of the lane's group pair inside a unit.
@@ -119,8 +135,8 @@ __device__ __forceinline__ void fill_codes(const BatchDev& b, const GroupPair& g
 #pragma unroll
   for (int g = 0; g < PAIRS_PER_UNIT; ++g) {
     int64_t hoff = b.hap_off[gp.hap];
-    hb[g] = __shfl(hoff, g * GROUP);
-    hm[g] = __shfl(gp.valid ? gp.m : 0, g * GROUP);
+    hb[g] = lane16(hoff, g);
+    hm[g] = lane16(gp.valid ? gp.m : 0, g);
   }
   for (int kk = lane; kk < nslots; kk += 64) {
     const int col = kk - 17;
@@ -169,11 +185,9 @@ __device__ __forceinline__ int decode_unit32(const BatchDev& b, const PlanDev& p
     if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
     gp.valid = 0;
   }
-  int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
-  mx = max(mx, __shfl_xor(mx, 16));
-  mx = max(mx, __shfl_xor(mx, 32));
-  mn = min(mn, __shfl_xor(mn, 16));
-  mn = min(mn, __shfl_xor(mn, 32));
+  const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+  const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+  const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
   if (mx == 0) return -1;
   gp.m_max = mx;
   gp.m_min = mn;

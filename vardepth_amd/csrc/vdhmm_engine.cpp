@@ -168,7 +168,7 @@ void tune_defaults() {
     g.tune.fwd_blocks_per_cu[r] = 0;
     g.tune.grad_variant[r] = 0;
     g.tune.grad_blocks_per_cu[r] = 0;
-    g.tune.fwd_share[r] = 3;
+    g.tune.fwd_share[r] = 2;
   }
   g.tune.share_chunk = 4;
   g.tune_init = true;

@@ -892,14 +892,14 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
 // the instance without spilling, found by compiling every instance at 2..7
 // waves.  1 = no request (the hint would not raise the occupancy, and it
 // changes the schedule).  Index: rows per lane R (0 unused).
-constexpr int WPE_T[2][2][17] = {{{1, 1, 6, 1, 1, 1, 1, 1, 1, 4, 1, 1, 1, 3, 3, 1, 1},
-                                  {1, 1, 1, 5, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
-                                 {{1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
-                                  {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}}};  // [anchored][suffix][R]
-constexpr int WPE_P[2][2][17] = {{{1, 8, 8, 1, 6, 1, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1},
-                                  {1, 8, 8, 7, 6, 5, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1}},
-                                 {{1, 1, 1, 7, 6, 1, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1},
-                                  {1, 1, 1, 7, 6, 5, 5, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1}}};  // [anchored][unroll 4][R]
+constexpr int WPE_T[2][2][17] = {{{1, 1, 1, 1, 1, 1, 1, 1, 1, 4, 4, 1, 1, 1, 3, 3, 1},
+                                  {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
+                                 {{1, 1, 6, 1, 5, 1, 1, 1, 1, 1, 3, 1, 1, 1, 1, 1, 1},
+                                  {1, 1, 1, 1, 1, 1, 1, 1, 3, 1, 1, 1, 1, 1, 1, 1, 1}}};  // [anchored][suffix][R]
+constexpr int WPE_P[2][2][17] = {{{1, 8, 8, 1, 1, 6, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+                                  {1, 8, 8, 8, 7, 6, 5, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}},
+                                 {{1, 8, 8, 1, 7, 6, 5, 1, 1, 4, 1, 1, 1, 3, 1, 1, 1},
+                                  {1, 8, 8, 7, 7, 6, 5, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1}}};  // [anchored][unroll 4][R]
 constexpr int wpe_t(bool anch, bool sfx, int r) { return WPE_T[anch][sfx][r]; }
 constexpr int wpe_p(bool anch, int u, int r) { return WPE_P[anch][u == 4][r]; }
 
@@ -922,7 +922,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_p(ANCH, 
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
-    u = __shfl(u, 0);
+    u = first_lane(u);
     if (u >= hi) break;
     GroupPair gp;
     if (decode_unit32(b, p, u, gp, out) != R) continue;
@@ -952,7 +952,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, 
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
-    u = __shfl(u, 0);
+    u = first_lane(u);
     if (u >= hi) break;
     ReadTask rt;
     if (decode_read_unit(b, p, u, rt) != R) continue;
@@ -978,8 +978,7 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   fill_codes(b, gp, codes, smax, tend + 18);
   const int np = gp.n + (ANCH ? 1 : 0);
   int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
-  int ntiles = max(ntile_g, __shfl_xor(ntile_g, 16));
-  ntiles = max(ntiles, __shfl_xor(ntiles, 32));
+  int ntiles = max(max(lane16(ntile_g, 0), lane16(ntile_g, 1)), max(lane16(ntile_g, 2), lane16(ntile_g, 3)));
   const int pad = ntiles * GROUP * R - np;
   const int64_t rb = b.read_off[gp.read];
   const double gam = tab->gamma, xx = tab->ext;
@@ -1128,7 +1127,7 @@ __global__ __launch_bounds__(64) void k_fwd64(BatchDev b, PlanDev p, const QTab<
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT64], 1ull);
-    u = __shfl(u, 0);
+    u = first_lane(u);
     if (u >= units) break;
     const unsigned long long idx = u * PAIRS_PER_UNIT + g;
     GroupPair gp;
@@ -1143,11 +1142,9 @@ __global__ __launch_bounds__(64) void k_fwd64(BatchDev b, PlanDev p, const QTab<
       if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
       gp.valid = 0;
     }
-    int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
-    mx = max(mx, __shfl_xor(mx, 16));
-    mx = max(mx, __shfl_xor(mx, 32));
-    mn = min(mn, __shfl_xor(mn, 16));
-    mn = min(mn, __shfl_xor(mn, 32));
+    const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+    const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+    const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
     if (mx == 0) continue;
     gp.m_max = mx;
     gp.m_min = mn;
