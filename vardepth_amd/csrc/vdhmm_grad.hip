@@ -213,9 +213,22 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
     wpair = weights ? weights[gp.out] : 1.0;
     seed = flag ? 0.0 : wpair / (double)term;
   }
-  const int src = (lane & ~15) + 15;
-  seed = __shfl(seed, src);
-  flag = __shfl((int)flag, src) != 0;
+  {  // the group's lane 15 holds the terminal: broadcast by readlane (scalar)
+    const int g = lane >> 4;
+    const int64_t sb = __builtin_bit_cast(int64_t, seed);
+    int64_t sv[PAIRS_PER_UNIT];
+    int fv[PAIRS_PER_UNIT];
+#pragma unroll
+    for (int q = 0; q < PAIRS_PER_UNIT; ++q) {
+      const int lo = __builtin_amdgcn_readlane((int)(uint32_t)sb, 16 * q + 15);
+      const int hi = __builtin_amdgcn_readlane((int)(sb >> 32), 16 * q + 15);
+      sv[q] = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+      fv[q] = __builtin_amdgcn_readlane((int)flag, 16 * q + 15);
+    }
+    const int64_t pick = g == 0 ? sv[0] : g == 1 ? sv[1] : g == 2 ? sv[2] : sv[3];
+    seed = __builtin_bit_cast(double, pick);
+    flag = (g == 0 ? fv[0] : g == 1 ? fv[1] : g == 2 ? fv[2] : fv[3]) != 0;
+  }
   const T tseed = (T)seed;
 
   // ---- sweep 2: backward, block by block
@@ -416,7 +429,7 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
-    u = __shfl(u, 0);
+    u = first_lane(u);
     if (u >= hi) break;
     GroupPair gp;
     if (decode_unit32(b, p, u, gp, out) != R) continue;
@@ -510,7 +523,7 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
   for (;;) {
     unsigned long long u = 0;
     if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT64], 1ull);
-    u = __shfl(u, 0);
+    u = first_lane(u);
     if (u >= units) break;
     const unsigned long long idx = u * PAIRS_PER_UNIT + g;
     GroupPair gp;
@@ -522,11 +535,9 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
     gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
     gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
     if (gp.valid && (gp.m == 0 || gp.n + (ANCH ? 1 : 0) > GROUP * R)) gp.valid = 0;
-    int mx = gp.valid ? gp.m : 0, mn = gp.valid ? gp.m : 0x7fffffff;
-    mx = max(mx, __shfl_xor(mx, 16));
-    mx = max(mx, __shfl_xor(mx, 32));
-    mn = min(mn, __shfl_xor(mn, 16));
-    mn = min(mn, __shfl_xor(mn, 32));
+    const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+    const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+    const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
     if (mx == 0) continue;
     gp.m_max = mx;
     gp.m_min = mn;
