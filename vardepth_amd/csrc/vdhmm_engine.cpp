@@ -463,7 +463,7 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   }
   int max64 = 1;
   for (int k = 0; k < 8; ++k) {
-    blocks64[k] = std::max(1, g.cus * std::max(1, std::min(g.gocc64[anchored][k], 2)));
+    blocks64[k] = std::max(1, g.cus * std::max(1, std::min(g.gocc64[anchored][k], 16)));
     max64 = std::max(max64, blocks64[k]);
   }
   max_blocks = std::max(max_blocks, max64);
