@@ -302,3 +302,29 @@ def test_suffix_continuation_matches_full_sweeps(mode):
     finally:
         L.vd_suffix_sharing(prev)
         autotune.set_tuning(base)
+
+
+def test_calls_on_different_streams_do_not_share_workspaces_concurrently():
+    """Forwards of two different batches enqueued back to back on two streams
+    with no synchronisation in between: the second call's stream waits for the
+    first call's work (they share the engine's plan and sweep workspaces), so
+    both results equal their eager values."""
+    import torch
+    a = engine.DeviceBatch(workloads.uneven_regions(seed=21, regions=96))
+    b = engine.DeviceBatch(workloads.chromosome_sample(mb=0.05, depth=30).batch)
+    a.forward(mode=0)
+    ref_a = a.result().copy()
+    b.forward(mode=0)
+    ref_b = b.result().copy()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        a.out.fill_(float("nan"))
+        b.out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        a.forward(mode=0, stream=s1)
+        b.forward(mode=0, stream=s2)
+        a.forward(mode=0, stream=s2)   # and the first batch again, after the second
+        torch.cuda.synchronize()
+        # FP64-rescued pairs may move in the last bit with the rescue list's order
+        np.testing.assert_allclose(a.result(), ref_a, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(b.result(), ref_b, rtol=1e-12, atol=0)

@@ -79,6 +79,8 @@ struct Engine {
   bool tune_init = false;
   bool class_timing = false;
   hipEvent_t ev_cls[17] = {};
+  hipEvent_t ev_last = nullptr;  // end of the previous call's device work
+  bool have_last = false;
   double fwd_cls_ms[17] = {}, grad_cls_ms[17] = {};
   int gocc_smax = -1;
   // haplotype prefix sharing of the FP32 forward
@@ -107,6 +109,7 @@ int engine_init() {
   HIPCHK(hipStreamCreateWithFlags(&g.own_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
   for (int i = 0; i < 17; ++i) HIPCHK(hipEventCreate(&g.ev_cls[i]));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_last, hipEventDisableTiming));
   g.dev = dev;
   return 0;
 }
@@ -133,6 +136,28 @@ void build_tab(const vd_params& p, vd::QTab<T>& t, int32_t* E) {
   t.gap = (T)p.gap;
   t.ext = (T)p.ext;
 }
+
+// Successive calls share the engine's device workspaces (plan, trie, suffix,
+// FP64 and genotype buffers, counters, parameter tables), and a call may run
+// on another stream than the previous one: its stream first waits for the
+// work the previous call enqueued.  Not under stream capture, where the
+// captured work is ordered by the graph and a wait on an event recorded
+// outside the capture is not allowed.
+struct StreamOrder {
+  hipStream_t st;
+  bool on = false;
+  explicit StreamOrder(hipStream_t s) : st(s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!g.ev_last || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    on = true;
+    if (g.have_last) (void)hipStreamWaitEvent(st, g.ev_last, 0);
+  }
+  ~StreamOrder() {
+    if (on && hipEventRecord(g.ev_last, st) == hipSuccess) g.have_last = true;
+  }
+  StreamOrder(const StreamOrder&) = delete;
+  StreamOrder& operator=(const StreamOrder&) = delete;
+};
 
 int check_params(const vd_params& p) {
   if (!(p.gap > 0 && p.gap < 0.5)) return fail("vdhmm: gap open must be in (0, 0.5)");
@@ -211,6 +236,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     return fail("vdhmm: negative batch size");
   if (int rc = check_params(*params)) return rc;
   if (int rc = engine_init()) return rc;
+  StreamOrder order(st);
   if (int rc = upload_params(*params, st)) return rc;
   const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
   const int force64 = precision == VD_PREC_FP64 ? 1 : 0;
@@ -385,6 +411,7 @@ int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, i
   if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
+  StreamOrder order(st);
   if (int rc = stage_host_batch(hb, npairs, st)) return rc;
   vd_batch db = staged_view(hb);
   vd_stats local;
@@ -435,6 +462,7 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (int rc = check_params(*params)) return rc;
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
+  StreamOrder order(st);
   if (int rc = upload_params(*params, st)) return rc;
   static vd::GradTab gt;
   build_grad_tab(*params, gt);
@@ -529,6 +557,7 @@ int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, cons
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
+  StreamOrder order(st);
   if (int rc = stage_host_batch(hb, npairs, st)) return rc;
   HIPCHK(g.in_grad.ensure(vd::NGRAD * sizeof(double)));
   const double* dw = nullptr;
@@ -566,6 +595,7 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
   if (int rc = check_params(*params)) return rc;
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
+  StreamOrder order(st);
   if (int rc = upload_params(*params, st)) return rc;
   if (int rc = stage_host_batch(hb, npairs, st)) return rc;
   static vd::QTab<float> t32;
@@ -703,6 +733,7 @@ int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t
   if (b->nregions > 0 && ((!lh && b->mat_size > 0) || (!gl && b->gl_size > 0)))
     return fail("vdhmm: null genotype matrix or output");
   if (int rc = engine_init()) return rc;
+  StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   const vd::GenoDev d = geno_dev(b);
   HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
@@ -724,6 +755,7 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   if (b->nregions > 0 && ((!ll && b->mat_size > 0) || (!mq && b->nrows > 0) || (!gl && b->gl_size > 0)))
     return fail("vdhmm: null genotype input or output");
   if (int rc = engine_init()) return rc;
+  StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   const vd::GenoDev d = geno_dev(b);
   HIPCHK(gg.mol.ensure(sizeof(double) * std::max<int64_t>(b->mat_size, 1)));
@@ -747,6 +779,7 @@ int guard_device_locked(const vd_geno_batch* b, const double* gl, const double* 
   if (std::isnan(min_cert) || !(margin_scale >= 0)) return fail("vdhmm: invalid guard margin or certainty bound");
   if (b->nregions > 0 && (!gl || !cert || !flags || !gaps)) return fail("vdhmm: null guard argument");
   if (int rc = engine_init()) return rc;
+  StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   if (int rc = geno_zero_ctr(st)) return rc;
   HIPCHK(vd::launch_guard(geno_dev(b), gl, cert, min_cert, margin_scale, flags, gaps,
@@ -792,6 +825,7 @@ int mixture_host_locked(int rows, int n, const double* lh, double prior, int tri
   if (!out) return fail("vdhmm: null output");
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
+  StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   const int64_t pairs = tri ? (int64_t)n * (n + 1) / 2 : (int64_t)n * n;
   const int64_t cnt = (int64_t)rows * n;
@@ -815,6 +849,7 @@ int normalized_host_locked(int rows, int n, const double* ll, const uint8_t* mq,
   if (!ll || !mq || !values || !cert) return fail("vdhmm: null argument");
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
+  StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   const int64_t cnt = (int64_t)rows * n;
   vd_geno_batch b;
