@@ -399,6 +399,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch choices)")
+    ap.add_argument("--tune-save", help="after tuning, keep the choice in this JSON file (autotune.save)")
+    ap.add_argument("--tune-load", help="instead of tuning, apply the choice kept in this JSON file")
     ap.add_argument("--no-forward", action="store_true",
                     help="skip the forward workload (profiling the fwd+bwd workload alone); prints no JSON line")
     ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
@@ -447,12 +449,20 @@ def main():
 
     dev = engine.DeviceBatch(batch)
     tune = None
-    if not args.no_tune:
+    tune_key = "%s-%d-mode%d-%s" % (args.workload, args.regions, args.mode, args.precision)
+    if args.tune_load:
+        from vardepth_amd import autotune
+        if not autotune.load(args.tune_load, tune_key):
+            raise SystemExit("no tuning for %s in %s" % (tune_key, args.tune_load))
+        tune = {"kind": "forward", "loaded": args.tune_load}
+    elif not args.no_tune:
         from vardepth_amd import autotune
         t0 = time.perf_counter()
         tune = autotune.tune_forward(dev, mode=args.mode, precision=args.precision)
         tune["seconds"] = round(time.perf_counter() - t0, 2)
         print("autotune forward: %s" % json.dumps(tune), file=sys.stderr, flush=True)
+        if args.tune_save:
+            autotune.save(args.tune_save, tune_key)
     for _ in range(args.warmup):
         dev.forward(mode=args.mode, precision=args.precision, stats=True)
     torch.cuda.synchronize()
