@@ -1110,8 +1110,10 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   __syncthreads();
 }
 
+// 4 waves per SIMD: the 6 values spilled at that occupancy are reloaded only
+// outside the sweep loops.
 template <bool ANCH>
-__global__ __launch_bounds__(64) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
                                               int smax, double* __restrict__ wsall, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
