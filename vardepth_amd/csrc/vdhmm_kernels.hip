@@ -804,7 +804,7 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
     // the tail, steps up to tend = J + 16 (lane 15 at column J last).  With the
     // suffix continuation the tail is steps J+1..J+16: columns <= J+15 < m, so
     // no sentinel codes and no latch; lane l is at column J after step
-    // J + l + 1 and leaves Q_i(J) and D^_i(J+1) in the workspace
+    // J + l + 1 and leaves Q_i(J), M^_i(J) and D^_i(J) in the workspace
     auto tail_snap = [&](int tt) {
       tail_step(tt);
       if (SFX && S && tt - l - 1 == J) {
