@@ -168,10 +168,12 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *   grad_blocks_per_cu  persistent adjoint grid per CU (0 = the occupancy limit)
  *   fwd_share           0: pair units; 1: haplotype prefix sharing for the class
  *                       (read units of share_chunk haplotypes); 2: prefix sharing
- *                       and the shared-suffix continuation; 3: 2 where the
- *                       suffix kernel keeps the occupancy of the prefix-only
- *                       kernel, else 1.  Applies while vd_prefix_sharing is on
- *                       (the suffix part while vd_suffix_sharing is on)
+ *                       and the shared-suffix continuation for every read unit;
+ *                       3: the suffix kernel for the read units of regions with
+ *                       a shared suffix, the prefix-only kernel (which holds
+ *                       more waves in some classes) for the others.  Applies
+ *                       while vd_prefix_sharing is on (the suffix part while
+ *                       vd_suffix_sharing is on)
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
  * Defaults: 2, 0, 0, 0, 2, 4.  Values never change results beyond rounding
  * order (FP64-rescued pairs may differ in the last bit; the suffix

@@ -276,10 +276,12 @@ def test_suffix_continuation_matches_full_sweeps(mode):
     base = autotune.get()
     try:
         t = autotune.get()
-        for chunk in (4, 2):
+        # the suffix kernel for every read unit (2), with units of 4 and 2
+        # haplotypes, and split with the prefix-only kernel by region (3)
+        for share, chunk in ((2, 4), (2, 2), (3, 4)):
             t.share_chunk = chunk
             for r in range(1, 17):
-                t.fwd_share[r] = 2
+                t.fwd_share[r] = share
             autotune.set_tuning(t)
             for i, b in enumerate(_suffix_cases()):
                 L.vd_suffix_sharing(0)

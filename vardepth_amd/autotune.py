@@ -31,10 +31,11 @@ import numpy as np
 from . import _lib
 
 # (unroll, blocks per CU (0 = occupancy limit), sharing: 0 pair units, 1 prefix, 2 prefix +
-# suffix, 3 suffix where it keeps the occupancy); the shared kernels have one unroll.  The
-# first is the default.
-FWD_CANDIDATES = [(2, 0, 2), (2, 0, 1), (2, 0, 3), (2, 0, 0), (4, 0, 0), (2, 3, 1), (2, 3, 2), (2, 3, 0), (4, 3, 0),
-                  (2, 2, 1), (2, 2, 2), (2, 2, 0)]
+# suffix for every read unit, 3 suffix kernel for regions with a shared suffix and the
+# prefix-only kernel for the others); the shared kernels have one unroll.  The first is the
+# default.
+FWD_CANDIDATES = [(2, 0, 2), (2, 0, 1), (2, 0, 3), (2, 0, 0), (4, 0, 0), (2, 3, 2), (2, 3, 1), (2, 3, 0), (4, 3, 0),
+                  (2, 2, 2), (2, 2, 1), (2, 2, 0)]
 CHUNK_CANDIDATES = (4, 8, 16)
 GRAD_CANDIDATES = [(0, 0), (1, 0)]                           # (variant, blocks per CU)
 

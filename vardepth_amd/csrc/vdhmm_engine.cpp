@@ -225,7 +225,7 @@ vd::PlanDev plan_dev(int64_t npairs, int class_major, int trie = 0) {
                      g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
                      class_major, trie, g.tune.share_chunk, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
                      g.trie_fork.as<int32_t>(), g.trie_nfork.as<int32_t>(), g.trie_ws.as<float>(),
-                     0, g.trie_sfx.as<int32_t>(), g.sfx_ws.as<float>()};
+                     0, 0, g.trie_sfx.as<int32_t>(), g.sfx_ws.as<float>()};
 }
 
 int forward_device_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
@@ -266,7 +266,9 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   int trie = 0;  // class mask of the read units with prefix sharing
   if (g.share && !force64)
     for (int r = 1; r <= vd::MAX_R32; ++r) trie |= g.tune.fwd_share[r] ? (1 << r) : 0;
-  int sfx = 0;  // class mask of the read units with the suffix continuation
+  int sfx = 0;    // class mask of the read units with the suffix continuation
+  int split = 0;  // of those, classes split between the suffix and the prefix-only kernel
+  int blocks32s[17] = {0};  // grid of the suffix kernel of a split class
   if (trie) {
     if (g.occt_smax != smax) {
       for (int a = 0; a < 2; ++a)
@@ -274,23 +276,29 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
           for (int x = 0; x < 2; ++x) HIPCHK(vd::occupancy_trie(a, r, x, smax, &g.occ32t[a][r][x]));
       g.occt_smax = smax;
     }
-    // fwd_share 2: always; 3 (default): where the suffix kernel keeps the
-    // occupancy of the prefix-only kernel (its code costs registers)
+    // fwd_share 2 (default): the suffix kernel takes every read unit of the
+    // class; 3: it takes the units of regions with a shared suffix and the
+    // prefix-only kernel, which holds more waves in some classes, the others
+    // (each kernel passes over the class's units and skips the other's)
     if (g.suffix)
       for (int r = 1; r <= vd::MAX_R32; ++r) {
         const int fs = g.tune.fwd_share[r];
-        const bool keep = g.occ32t[anchored][r][1] >= g.occ32t[anchored][r][0];
-        sfx |= (fs == 2 || (fs == 3 && keep)) ? (1 << r) : 0;
+        sfx |= fs >= 2 ? (1 << r) : 0;
+        split |= fs == 3 ? (1 << r) : 0;
       }
     size_t ws_floats = 0, sfx_floats = 0;
     for (int r = 1; r <= vd::MAX_R32; ++r) {
       if (!((trie >> r) & 1)) continue;
-      const int occ = std::max(1, g.occ32t[anchored][r][(sfx >> r) & 1]);
+      const bool s_only = ((sfx >> r) & 1) && !((split >> r) & 1);
       const int want = g.tune.fwd_blocks_per_cu[r];
+      const int occ = std::max(1, g.occ32t[anchored][r][s_only ? 1 : 0]);
       blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
-      ws_floats = std::max(ws_floats, (size_t)blocks32[r] * vd::PAIRS_PER_UNIT * vd::TRIE_SLOTS * vd::GROUP *
+      const int occs = std::max(1, g.occ32t[anchored][r][1]);
+      blocks32s[r] = (split >> r) & 1 ? g.cus * (want > 0 ? std::min(want, occs) : occs) : 0;
+      const int maxb = std::max(blocks32[r], blocks32s[r]);
+      ws_floats = std::max(ws_floats, (size_t)maxb * vd::PAIRS_PER_UNIT * vd::TRIE_SLOTS * vd::GROUP *
                                           ((3 * r + 3 + 3) & ~3));
-      sfx_floats = std::max(sfx_floats, (size_t)blocks32[r] * vd::SFX_WS);
+      sfx_floats = std::max(sfx_floats, (size_t)(s_only ? blocks32[r] : blocks32s[r]) * vd::SFX_WS);
     }
     const int64_t nh = std::max<int64_t>(db->nhaps, 1), nreg = std::max<int64_t>(db->nregions, 1);
     HIPCHK(g.trie_order.ensure(nh * 4));
@@ -304,9 +312,11 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   const vd::BatchDev b = batch_dev(db);
   vd::PlanDev p = plan_dev(npairs, 1, trie);
   p.sfx = sfx;
+  p.split = split;
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
-                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, unroll32, blocks64,
+                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks32s, unroll32,
+                             blocks64,
                              g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr, timing ? g.ev_cls : nullptr));
   if (timing && !force64) {
     HIPCHK(hipEventSynchronize(g.ev[3]));

@@ -107,6 +107,8 @@ struct PlanDev {
   // each haplotype is then swept to column m - S only and its terminal
   // completed by a dot product with the adjoint
   int sfx;                  // bit c set: class c runs with the suffix continuation (planner finds S if any)
+  int split;                // bit c set (with sfx): the suffix kernel takes only the read units of regions
+                            // with a shared suffix, the prefix-only kernel the others
   int32_t* trie_sfx;        // [nregions] common suffix S (0: none)
   float* sfx_ws;            // [block][64 lanes][84]: adjoint weights, boundary mass, the column m - S state
 };
@@ -127,7 +129,8 @@ enum {
   CTR_SWEPT = 7,    // FP32 cells swept (read rows x haplotype columns computed)
   CTR_CLS = 8,      // [8 + c]: next unit of class c (per-class gradient kernels)
   CTR_BIN64 = 32,   // [32 + b]: FP64 gradient sub-list lengths (8 row bins)
-  CTR_COUNT = 40
+  CTR_CLS2 = 40,    // [40 + c]: next unit of class c for the second kernel of a split class
+  CTR_COUNT = 57
 };
 
 // Device view of a genotype batch: region k owns rows row_off[k]..row_off[k+1]
@@ -171,6 +174,7 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
                  hipStream_t st);  // p.class_major selects the bucket order
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
+                           const int* blocks32s,
                            const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
                            hipEvent_t* ev, hipEvent_t* ev_cls);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,

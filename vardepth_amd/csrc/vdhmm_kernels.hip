@@ -936,9 +936,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_p(ANCH, 
 // The prefix-sharing forward of class R over the read units of the plan (SFX:
 // with the suffix continuation, a separate instantiation because its code
 // raises the register count of some classes).
+//
+// part 0 takes every read unit of the class; with a split class, part 1 (the
+// SFX instance) takes the units whose region has a shared suffix and whose
+// chunk holds two or more haplotypes, part 2 (the prefix-only instance) the
+// others, from a counter of its own.
 template <bool ANCH, int R, int U, bool SFX>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, SFX, R)))) void k_fwd32t(BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab,
-                                               int smax, double* __restrict__ out) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, SFX, R)))) void k_fwd32t(
+    BatchDev b, PlanDev p, const QTab<float>* __restrict__ gtab, int smax, double* __restrict__ out, int part) {
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
   const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   if (lo >= hi) return;
@@ -952,13 +957,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, 
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
+  unsigned long long* next = &p.ctr[(part == 2 ? CTR_CLS2 : CTR_CLS) + R];
   for (;;) {
     unsigned long long u = 0;
-    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
+    if (lane == 0) u = lo + atomicAdd(next, 1ull);
     u = first_lane(u);
     if (u >= hi) break;
     ReadTask rt;
     if (decode_read_unit(b, p, u, rt) != R) continue;
+    if (part) {  // uniform: one region and chunk per unit
+      const bool sfx_unit = p.trie_sfx[rt.region] > 0 && rt.q1 - rt.q0 >= 2;
+      if (sfx_unit != (part == 1)) continue;
+    }
     unit32_trie<R, ANCH, U, SFX>(b, p, tab, codes, smax, rt, out);
   }
 }
@@ -1238,11 +1248,11 @@ hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_pe
 
 template <bool ANCH, bool SFX>
 static void launch_fwd32t_r(int r, int blocks, size_t lds, hipStream_t st, const BatchDev& b, const PlanDev& p,
-                            const QTab<float>* tab32, int smax, double* out) {
+                            const QTab<float>* tab32, int smax, double* out, int part) {
   switch (r) {
-#define VD_L(RR)                                                                                          \
-  case RR:                                                                                                \
-    hipLaunchKernelGGL((k_fwd32t<ANCH, RR, 2, SFX>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out); \
+#define VD_L(RR)                                                                                                  \
+  case RR:                                                                                                        \
+    hipLaunchKernelGGL((k_fwd32t<ANCH, RR, 2, SFX>), dim3(blocks), dim3(64), lds, st, b, p, tab32, smax, out, part); \
     break;
     VD_L(1) VD_L(2) VD_L(3) VD_L(4) VD_L(5) VD_L(6) VD_L(7) VD_L(8)
     VD_L(9) VD_L(10) VD_L(11) VD_L(12) VD_L(13) VD_L(14) VD_L(15) VD_L(16)
@@ -1272,6 +1282,7 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
+                           const int* blocks32s,
                            const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
                            hipEvent_t* ev, hipEvent_t* ev_cls) {
   if (ev) (void)hipEventRecord(ev[0], st);
@@ -1283,12 +1294,15 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
       const bool u4 = unroll32[r] == 4;
       if ((p.trie >> r) & 1) {
         const size_t lds = lds_fwd32(smax);
-        if ((p.sfx >> r) & 1) {
-          if (anchored) launch_fwd32t_r<true, true>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
-          else launch_fwd32t_r<false, true>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
-        } else {
-          if (anchored) launch_fwd32t_r<true, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
-          else launch_fwd32t_r<false, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out);
+        const bool sfx = (p.sfx >> r) & 1, split = sfx && ((p.split >> r) & 1);
+        if (sfx) {  // all units, or (split) those of regions with a shared suffix
+          const int blocks = split ? blocks32s[r] : blocks32[r];
+          if (anchored) launch_fwd32t_r<true, true>(r, blocks, lds, st, b, p, tab32, smax, out, split ? 1 : 0);
+          else launch_fwd32t_r<false, true>(r, blocks, lds, st, b, p, tab32, smax, out, split ? 1 : 0);
+        }
+        if (!sfx || split) {  // all units, or (split) the others
+          if (anchored) launch_fwd32t_r<true, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out, split ? 2 : 0);
+          else launch_fwd32t_r<false, false>(r, blocks32[r], lds, st, b, p, tab32, smax, out, split ? 2 : 0);
         }
       } else if (anchored) {
         if (u4) launch_fwd32_r<true, 4>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
