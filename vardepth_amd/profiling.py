@@ -16,6 +16,8 @@ runs, so FETCH_SIZE / WRITE_SIZE can be scaled to bytes per dispatch.
     python -m vardepth_amd.profiling traffic FETCH_DB WRITE_DB [CALIB_DB]
         HBM bytes per dispatch per kernel, each counter scaled by what it
         reads for the 256 MiB copy (taken from CALIB_DB when given)
+    python -m vardepth_amd.profiling counters PMC_DB
+        per kernel, every counter of the pass summed over its dispatches
     python -m vardepth_amd.profiling passes KT_DB N
         the last N passes of the batch (each starts at a k_prep_reads
         dispatch): per kernel, dispatches, average and total duration, i.e.
@@ -147,7 +149,29 @@ def last_passes(db: str, n: int):
     return [(r[0], r[1], r[2] / 1e6, r[3] / 1e6) for r in rows]
 
 
+def counter_table(db: str):
+    """(counter names, [(kernel, dispatches, {counter: sum})]) sorted by kernel time."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select k.name, p.counter_name, sum(p.counter_value), count(distinct k.dispatch_id), "
+                       "sum(k.duration) from pmc_events p join kernels k on k.dispatch_id = p.dispatch_id "
+                       "group by k.name, p.counter_name").fetchall()
+    names = sorted({r[1] for r in rows})
+    table = {}
+    for name, counter, value, n, dur in rows:
+        e = table.setdefault(name, {"n": n, "dur": 0.0, "c": {}})
+        e["c"][counter] = value
+        e["dur"] = max(e["dur"], dur)
+    order = sorted(table.items(), key=lambda kv: -kv[1]["dur"])
+    return names, [(k, v["n"], v["c"]) for k, v in order]
+
+
 def main(argv):
+    if len(argv) >= 2 and argv[0] == "counters":
+        names, rows = counter_table(argv[1])
+        print("%-60s %5s " % ("kernel", "calls") + " ".join("%14s" % n[-14:] for n in names))
+        for k, n, c in rows:
+            print("%-60s %5d " % (k[:60], n) + " ".join("%14.4g" % c.get(x, 0) for x in names))
+        return 0
     if len(argv) >= 3 and argv[0] == "passes":
         print("%-72s %6s %10s %10s" % ("kernel", "calls", "avg ms", "total ms"))
         for r in last_passes(argv[1], int(argv[2])):
