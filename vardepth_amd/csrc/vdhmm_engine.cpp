@@ -658,7 +658,7 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
 // ---- diploid genotype likelihood ------------------------------------------
 
 struct GenoBufs {
-  DevBuf mol, keep, units, map, ctr, mq_err;
+  DevBuf mol, keep, units, map, ctr, mq_err, kidx, nkept;
   DevBuf in_off, in_lh, in_gl, in_mq, in_vals, in_cert;  // host entry points
   bool mq_err_ready = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -750,8 +750,11 @@ int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t
   HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
   if (int rc = geno_zero_ctr(st)) return rc;
   if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
-  HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, gg.units.as<int64_t>(), gg.map.as<int32_t>(), gl,
-                            gg.ctr.as<unsigned long long>(), g.cus, st));
+  HIPCHK(gg.kidx.ensure(sizeof(int32_t) * std::max<int64_t>(b->nrows, 1)));
+  HIPCHK(gg.nkept.ensure(sizeof(int32_t) * std::max<int64_t>(b->nregions, 1)));
+  HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, gg.units.as<int64_t>(), gg.map.as<int32_t>(),
+                            gg.kidx.as<int32_t>(), gg.nkept.as<int32_t>(), gl, gg.ctr.as<unsigned long long>(),
+                            g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
   return geno_stats(st, s, false);
 }
@@ -777,8 +780,11 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   HIPCHK(vd::launch_molecule(d, ll, mq, gg.mq_err.as<double>(), min_cert, values, cert, gg.mol.as<double>(),
                              gg.keep.as<int8_t>(), gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
+  HIPCHK(gg.kidx.ensure(sizeof(int32_t) * std::max<int64_t>(b->nrows, 1)));
+  HIPCHK(gg.nkept.ensure(sizeof(int32_t) * std::max<int64_t>(b->nregions, 1)));
   HIPCHK(vd::launch_mixture(d, gg.mol.as<double>(), gg.keep.as<int8_t>(), prior, 1, gg.units.as<int64_t>(),
-                            gg.map.as<int32_t>(), gl, gg.ctr.as<unsigned long long>(), g.cus, st));
+                            gg.map.as<int32_t>(), gg.kidx.as<int32_t>(), gg.nkept.as<int32_t>(), gl,
+                            gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
   return geno_stats(st, s, true);
 }

@@ -133,6 +133,34 @@ __global__ __launch_bounds__(256) void k_molecule(GenoDev g, const double* __res
 }
 
 // ---------------------------------------------------------------------------
+// kept rows of every region, in row order (one wave per region): the mixture
+// loops then visit only the rows the certainty filter kept, with no branch on
+// a keep load per row (at 30x most rows are dropped)
+
+__global__ __launch_bounds__(256) void k_keep_compact(GenoDev g, const int8_t* __restrict__ keep,
+                                                      int32_t* __restrict__ kidx, int32_t* __restrict__ nkept) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t k = wave; k < g.nregions; k += nwaves) {
+    const int64_t r0 = g.row_off[k], nr = g.row_off[k + 1] - r0;
+    if (r0 < 0 || nr < 0 || r0 + nr > g.nrows) {  // uniform; k_mixture reports the layout
+      if (lane == 0) nkept[k] = 0;
+      continue;
+    }
+    int base = 0;
+    for (int64_t c = 0; c < nr; c += 64) {
+      const int64_t r = c + lane;
+      const bool kp = r < nr && keep[r0 + r];
+      const unsigned long long m = __builtin_amdgcn_ballot_w64(kp);
+      if (kp) kidx[r0 + base + __builtin_popcountll(m & ((1ull << lane) - 1))] = (int32_t)r;
+      base += __builtin_popcountll(m);
+    }
+    if (lane == 0) nkept[k] = base;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // wave units over pairs
 
 __global__ void k_mix_count(GenoDev g, int triangular, int64_t* __restrict__ unit_cnt) {
@@ -198,6 +226,7 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
                                                  const int8_t* __restrict__ keep, double prior, int triangular,
                                                  const int64_t* __restrict__ unit_off,
                                                  const int32_t* __restrict__ unit_region,
+                                                 const int32_t* __restrict__ kidx, const int32_t* __restrict__ nkept,
                                                  double* __restrict__ gl, unsigned long long* __restrict__ ctr) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
@@ -236,12 +265,15 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
       continue;
     }
 
+    // the kept rows in row order: kidx[r0 .. r0 + nk) (all rows without a keep mask)
+    const int64_t nk = keep ? nkept[k] : nr;
+    auto row_at = [&](int64_t i) { return M + (keep ? (int64_t)kidx[r0 + i] : i) * n; };
+
     if (prior < 0) {  // the pair's mean, no mixture (mixture_general.cpp:23)
       LogProd t;
       t.init();
-      for (int64_t r = 0; r < nr; ++r) {
-        if (keep && !keep[r0 + r]) continue;
-        const double* row = M + r * n;
+      for (int64_t i = 0; i < nk; ++i) {
+        const double* row = row_at(i);
         t.mul(floor300((row[a] + row[b]) / 2));
       }
       if (active) out[q] = t.total();
@@ -252,9 +284,8 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
     LogProd tb[7];
 #pragma unroll
     for (int j = 0; j < 7; ++j) tb[j].init();
-    for (int64_t r = 0; r < nr; ++r) {
-      if (keep && !keep[r0 + r]) continue;
-      const double* row = M + r * n;
+    for (int64_t i = 0; i < nk; ++i) {
+      const double* row = row_at(i);
       const double la = row[a], lb = row[b];
 #pragma unroll
       for (int j = 0; j < 7; ++j) tb[j].mul(floor300(G_BAL[j] * la + (1 - G_BAL[j]) * lb));
@@ -278,9 +309,8 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
         LogProd ta[MIXW];
 #pragma unroll
         for (int i = 0; i < MIXW; ++i) ta[i].init();
-        for (int64_t r = 0; r < nr; ++r) {
-          if (keep && !keep[r0 + r]) continue;
-          const double* row = M + r * n;
+        for (int64_t i = 0; i < nk; ++i) {
+          const double* row = row_at(i);
           const double half = om * (row[a] + row[b]) / 2;
 #pragma unroll
           for (int i = 0; i < MIXW; ++i) {
@@ -373,17 +403,23 @@ hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq
 }
 
 hipError_t launch_mixture(const GenoDev& g, const double* mat, const int8_t* keep, double prior, int triangular,
-                          int64_t* unit_off, int32_t* unit_region, double* gl, unsigned long long* ctr, int cus,
-                          hipStream_t st) {
+                          int64_t* unit_off, int32_t* unit_region, int32_t* kidx, int32_t* nkept, double* gl,
+                          unsigned long long* ctr, int cus, hipStream_t st) {
   if (g.nregions <= 0) return hipSuccess;
   const int cb = (int)((g.nregions + 255) / 256 < 4096 ? (g.nregions + 255) / 256 : 4096);
+  if (keep) {
+    const int64_t want = (g.nregions + 3) / 4;  // 4 waves (regions) per block
+    const int kb = (int)(want < (int64_t)cus * 32 ? want : (int64_t)cus * 32);
+    hipLaunchKernelGGL(k_keep_compact, dim3(kb), dim3(256), 0, st, g, keep, kidx, nkept);
+  }
   hipLaunchKernelGGL(k_mix_count, dim3(cb), dim3(256), 0, st, g, triangular, unit_off);
   hipLaunchKernelGGL(k_mix_scan, dim3(1), dim3(1024), 0, st, unit_off, g.nregions, ctr);
   // a unit count beyond the map's capacity (a layout whose blocks overlap) is
   // clipped by the map and the mixture; the host reads GCTR_UNITS to report it
   hipLaunchKernelGGL(k_mix_map, dim3(cb), dim3(256), 0, st, g, (const int64_t*)unit_off, unit_region);
   hipLaunchKernelGGL(k_mixture, dim3(cus * 8), dim3(256), 0, st, g, mat, keep, prior, triangular,
-                     (const int64_t*)unit_off, (const int32_t*)unit_region, gl, ctr);
+                     (const int64_t*)unit_off, (const int32_t*)unit_region, (const int32_t*)kidx,
+                     (const int32_t*)nkept, gl, ctr);
   return hipGetLastError();
 }
 

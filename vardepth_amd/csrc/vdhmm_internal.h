@@ -195,6 +195,6 @@ hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq
                            double min_cert, double* values, double* cert, double* mol, int8_t* keep,
                            unsigned long long* ctr, int cus, hipStream_t st);
 hipError_t launch_mixture(const GenoDev& g, const double* mat, const int8_t* keep, double prior, int triangular,
-                          int64_t* unit_off, int32_t* unit_region, double* gl, unsigned long long* ctr, int cus,
-                          hipStream_t st);
+                          int64_t* unit_off, int32_t* unit_region, int32_t* kidx, int32_t* nkept, double* gl,
+                          unsigned long long* ctr, int cus, hipStream_t st);
 }  // namespace vd
