@@ -1048,6 +1048,22 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     double* oP = bufP[tile & 1];
     double* oQ = bufQ[tile & 1];
     const double C = 0x1p500;
+    // The previous tile's last row enters lane 0 at step t as column t-1
+    // (slot t+16; zero past m_max).  It is read 16 steps per vector load:
+    // from step t0 (a multiple of 16) lane l holds step t0+l's values, and
+    // they shift down one lane per step; the next 16 are loaded one chunk
+    // ahead, so no step waits on a load.
+    double bP = 0.0, bQ = 0.0, nP = 0.0, nQ = 0.0;
+    if (!first) {
+      if (l - 1 <= gp.m_max) {
+        bP = inP[l + 16];
+        bQ = inQ[l + 16];
+      }
+      if (l + 15 <= gp.m_max) {
+        nP = inP[l + 32];
+        nQ = inQ[l + 32];
+      }
+    }
     for (int t = 0; t <= tend; ++t) {
       const uint32_t code = cl[t];
       double pinj, qinj;
@@ -1059,10 +1075,19 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
           qinj = t == 0 ? 0.0 : C;
         }
       } else {
-        // column t-1 of the previous tile's last row (slot t+16); none past m_max
-        const bool have = t - 1 <= gp.m_max;
-        pinj = have ? inP[t + 16] * sc1 * sc2 : 0.0;
-        qinj = have ? inQ[t + 16] * sc1 * sc2 : 0.0;
+        if (t > 0 && (t & 15) == 0) {
+          bP = nP;
+          bQ = nQ;
+          nP = nQ = 0.0;
+          if (t + l + 15 <= gp.m_max) {
+            nP = inP[t + l + 32];
+            nQ = inQ[t + l + 32];
+          }
+        }
+        pinj = bP * sc1 * sc2;
+        qinj = bQ * sc1 * sc2;
+        bP = dpp_shl1(0.0, bP);
+        bQ = dpp_shl1(0.0, bQ);
       }
       const double Pin = dpp_shr1(pinj, Plast);
       const double Xn = dpp_shr1(qinj, Q[R - 1]);
@@ -1120,10 +1145,11 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   __syncthreads();
 }
 
-// 4 waves per SIMD: the 6 values spilled at that occupancy are reloaded only
-// outside the sweep loops.
+// 3 waves per SIMD, no spills (158-162 VGPRs with the boundary-row conveyor).
+// At 4 waves the conveyor's registers spill inside the sweep and the kernel
+// measured slower: 57.1 vs 55.6 ms on the 48,592-pair precision-stress batch.
 template <bool ANCH>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
                                               int smax, double* __restrict__ wsall, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
