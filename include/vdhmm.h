@@ -207,6 +207,16 @@ int vd_prefix_sharing(int on);
  * reorder the sums).  on = 1 / 0 sets it, on < 0 only queries; returns the
  * previous setting. */
 int vd_suffix_sharing(int on);
+/* Launch option with no counterpart in the reference (its CPU path has no
+ * streams): when on (the default; VARDEPTH_CLASS_OVERLAP=0 in the environment
+ * at start-up turns it off), consecutive FP32 class kernels alternate between
+ * the caller's stream and a second stream of the library, forked from and
+ * joined back into the caller's stream within the call, so that one
+ * persistent kernel's last units overlap the next kernel's first.  Results
+ * are identical either way.  Calls that time each class (vd_class_timing)
+ * keep one stream.  on = 1 / 0 sets it, on < 0 only queries; returns the
+ * previous setting. */
+int vd_class_overlap(int on);
 /* When on, calls made with a stats pointer also time each class kernel;
  * vd_class_ms returns the last such times (17 each, index = class). */
 void vd_class_timing(int on);

@@ -152,6 +152,49 @@ def test_plain_baseline_matches():
     close(out1, G["grid_dense_m1"][short], TOL_MIXED)
 
 
+def test_class_overlap_keeps_results():
+    """Consecutive class kernels on two streams (the default) give the same
+    likelihoods as one stream, bit for bit outside the FP64-rescued pairs, in
+    modes 0/1/2 with prefix sharing and the suffix continuation, with pair
+    units, and from a captured graph."""
+    import torch
+    L = _lib.lib()
+    prev = L.vd_class_overlap(-1)
+    prev_share = L.vd_prefix_sharing(-1)
+    try:
+        b = workloads.uneven_regions(seed=12, regions=48)
+        dev = engine.DeviceBatch(b)
+        for share in (1, 0):
+            L.vd_prefix_sharing(share)
+            for mode in (0, 1, 2):
+                L.vd_class_overlap(0)
+                dev.forward(mode=mode)
+                ref = dev.result()
+                L.vd_class_overlap(1)
+                st = dev.forward(mode=mode, stats=True)
+                got = dev.result()
+                assert np.array_equal(np.isnan(got), np.isnan(ref))
+                differ = np.flatnonzero((got != ref) & ~np.isnan(ref))
+                assert len(differ) <= st["fp64_pairs"]
+                np.testing.assert_allclose(got[differ], ref[differ], rtol=1e-13, atol=0)
+        L.vd_prefix_sharing(1)
+        L.vd_class_overlap(0)
+        dev.forward(mode=0)
+        ref = dev.result()
+        L.vd_class_overlap(1)
+        g = dev.graph(mode=0)
+        for _ in range(3):
+            g.replay()
+            torch.cuda.synchronize()
+            got = dev.result()
+            differ = np.flatnonzero((got != ref) & ~np.isnan(ref))
+            np.testing.assert_allclose(got[differ], ref[differ], rtol=1e-13, atol=0)
+            dev.forward(mode=0)   # eager work between replays
+    finally:
+        L.vd_class_overlap(prev)
+        L.vd_prefix_sharing(prev_share)
+
+
 def test_autotune_keeps_results():
     """Every launch choice gives the same likelihoods and gradients (with the
     suffix continuation off: it agrees to FP32 rounding, tested on its own)."""

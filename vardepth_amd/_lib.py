@@ -31,7 +31,7 @@ SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch
            "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_tuning_get",
            "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_mixture_general", "vd_mixture_two",
            "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_genotype_guard", "vd_prefix_sharing",
-           "vd_suffix_sharing",
+           "vd_suffix_sharing", "vd_class_overlap",
            "vd_last_error", "vd_device")
 
 
@@ -149,6 +149,8 @@ def lib() -> ctypes.CDLL:
     L.vd_prefix_sharing.restype = ctypes.c_int
     L.vd_suffix_sharing.argtypes = [ctypes.c_int]
     L.vd_suffix_sharing.restype = ctypes.c_int
+    L.vd_class_overlap.argtypes = [ctypes.c_int]
+    L.vd_class_overlap.restype = ctypes.c_int
     L.vd_class_timing.argtypes = [ctypes.c_int]
     L.vd_class_timing.restype = None
     L.vd_class_ms.argtypes = [_dp, _dp]

@@ -89,6 +89,11 @@ struct Engine {
   int occ32t[2][17][2] = {};  // [anchored][class][suffix continuation]
   int occt_smax = -1;
   DevBuf trie_order, trie_start, trie_fork, trie_nfork, trie_ws, trie_sfx, sfx_ws;
+  // consecutive FP32 class kernels alternate between the caller's stream and
+  // this one (vd_class_overlap; VARDEPTH_CLASS_OVERLAP=0 at start-up keeps one)
+  bool overlap = true;
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 Engine g;
@@ -110,6 +115,11 @@ int engine_init() {
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
   for (int i = 0; i < 17; ++i) HIPCHK(hipEventCreate(&g.ev_cls[i]));
   HIPCHK(hipEventCreateWithFlags(&g.ev_last, hipEventDisableTiming));
+  HIPCHK(hipStreamCreateWithFlags(&g.side_stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
+  const char* ov = std::getenv("VARDEPTH_CLASS_OVERLAP");
+  if (ov) g.overlap = ov[0] != '0';
   g.dev = dev;
   return 0;
 }
@@ -269,6 +279,9 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   int sfx = 0;    // class mask of the read units with the suffix continuation
   int split = 0;  // of those, classes split between the suffix and the prefix-only kernel
   int blocks32s[17] = {0};  // grid of the suffix kernel of a split class
+  vd::SideLane lane2{g.side_stream, g.ev_fork, g.ev_join, 0, 0};
+  const bool two = g.overlap && g.side_stream;
+  const size_t halves = two ? 2 : 1;  // the side stream's kernels use the second half
   if (trie) {
     if (g.occt_smax != smax) {
       for (int a = 0; a < 2; ++a)
@@ -305,9 +318,11 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     HIPCHK(g.trie_start.ensure(nh * 4));
     HIPCHK(g.trie_fork.ensure(nh * 4));
     HIPCHK(g.trie_nfork.ensure(nreg * 4));
-    HIPCHK(g.trie_ws.ensure(ws_floats * sizeof(float)));
+    HIPCHK(g.trie_ws.ensure(halves * ws_floats * sizeof(float)));
     HIPCHK(g.trie_sfx.ensure(nreg * 4));
-    if (sfx) HIPCHK(g.sfx_ws.ensure(sfx_floats * sizeof(float)));
+    if (sfx) HIPCHK(g.sfx_ws.ensure(halves * sfx_floats * sizeof(float)));
+    lane2.trie_half = ws_floats;
+    lane2.sfx_half = sfx ? sfx_floats : 0;
   }
   const vd::BatchDev b = batch_dev(db);
   vd::PlanDev p = plan_dev(npairs, 1, trie);
@@ -317,7 +332,8 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks32s, unroll32,
                              blocks64,
-                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr, timing ? g.ev_cls : nullptr));
+                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr, timing ? g.ev_cls : nullptr,
+                             two ? &lane2 : nullptr));
   if (timing && !force64) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     for (int r = 1; r <= vd::MAX_R32; ++r) {
@@ -1144,6 +1160,13 @@ int vd_suffix_sharing(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   const int was = g.suffix ? 1 : 0;
   if (on >= 0) g.suffix = on != 0;
+  return was;
+}
+
+int vd_class_overlap(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int was = g.overlap ? 1 : 0;
+  if (on >= 0) g.overlap = on != 0;
   return was;
 }
 

@@ -172,11 +172,20 @@ hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu); 
 hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
                  hipStream_t st);  // p.class_major selects the bucket order
+// A second stream for the FP32 class kernels: consecutive classes alternate
+// between the caller's stream and `side`, so that the last units of one
+// persistent kernel overlap the first of the next.  Kernels on `side` use the
+// second half of the prefix-sharing workspaces (offsets in floats).
+struct SideLane {
+  hipStream_t side;
+  hipEvent_t fork, join;
+  size_t trie_half, sfx_half;
+};
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            const int* blocks32s,
                            const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
-                           hipEvent_t* ev, hipEvent_t* ev_cls);
+                           hipEvent_t* ev, hipEvent_t* ev_cls, const SideLane* lane2);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
 size_t lds_grad32(int smax);

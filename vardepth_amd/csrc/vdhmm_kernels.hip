@@ -1309,14 +1309,28 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            const int* blocks32s,
-                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
-                           hipEvent_t* ev, hipEvent_t* ev_cls) {
+                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st0,
+                           hipEvent_t* ev, hipEvent_t* ev_cls, const SideLane* lane2) {
+  hipStream_t st = st0;
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, force64, out, st);
   if (ev) (void)hipEventRecord(ev[1], st);
   if (!force64) {
+    // per-class timing keeps one stream, so that each class is timed alone
+    const bool two = lane2 && !ev_cls;
+    const PlanDev& p0 = p;
+    PlanDev p1 = p;
+    if (two) {
+      if (p1.trie_ws) p1.trie_ws += lane2->trie_half;
+      if (p1.sfx_ws) p1.sfx_ws += lane2->sfx_half;
+      (void)hipEventRecord(lane2->fork, st0);
+      (void)hipStreamWaitEvent(lane2->side, lane2->fork, 0);
+    }
     for (int r = MAX_R32; r >= 1; --r) {  // largest class first
       if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
+      const bool alt = two && (r & 1);
+      const PlanDev& p = alt ? p1 : p0;
+      st = alt ? lane2->side : st0;
       const bool u4 = unroll32[r] == 4;
       if ((p.trie >> r) & 1) {
         const size_t lds = lds_fwd32(smax);
@@ -1338,7 +1352,12 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
         else launch_fwd32_r<false, 2>(r, blocks32[r], lds_fwd32(smax), st, b, p, tab32, smax, out);
       }
     }
+    st = st0;
     if (ev_cls) (void)hipEventRecord(ev_cls[0], st);
+    if (two) {
+      (void)hipEventRecord(lane2->join, lane2->side);
+      (void)hipStreamWaitEvent(st0, lane2->join, 0);
+    }
   }
   if (ev) (void)hipEventRecord(ev[2], st);
   if (anchored)

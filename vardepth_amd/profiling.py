@@ -22,6 +22,10 @@ runs, so FETCH_SIZE / WRITE_SIZE can be scaled to bytes per dispatch.
         the last N passes of the batch (each starts at a k_prep_reads
         dispatch): per kernel, dispatches, average and total duration, i.e.
         the timed passes after an autotuning sweep, without its candidates
+    python -m vardepth_amd.profiling spans KT_DB N
+        the last N passes: span of the FP32 class kernels (first start to
+        last end) and the sum of their durations; the two differ when
+        consecutive classes overlap on two streams
     python -m vardepth_amd.profiling families KT_DB [FETCH_DB WRITE_DB]
         kernels grouped by family (the template instances of one kernel, e.g.
         the per-class k_fwd32<A, R, U>): variants, dispatches per variant, the
@@ -149,6 +153,26 @@ def last_passes(db: str, n: int):
     return [(r[0], r[1], r[2] / 1e6, r[3] / 1e6) for r in rows]
 
 
+def class_spans(db: str, n: int):
+    """[(span_ms, sum_ms, dispatches)] of the FP32 class kernels in each of the
+    last n forward passes (a pass runs from one k_prep_reads to the next).
+    The span runs from the first class kernel's start to the last one's end;
+    with consecutive classes on two streams it is the class kernels' time,
+    and the sum of their durations counts the overlapped time twice."""
+    con = sqlite3.connect(db)
+    starts = [r[0] for r in con.execute("select start from kernels where name like '%k_prep_reads%' "
+                                        "order by start").fetchall()]
+    out = []
+    for i in range(max(0, len(starts) - int(n)), len(starts)):
+        cond = "start >= ?" + (" and start < ?" if i + 1 < len(starts) else "")
+        args = (starts[i], starts[i + 1]) if i + 1 < len(starts) else (starts[i],)
+        r = con.execute("select min(start), max(start + duration), sum(duration), count(*) from kernels where "
+                        + cond + " and (name like '%k_fwd32<%' or name like '%k_fwd32t<%')", args).fetchone()
+        if r[3]:
+            out.append(((r[1] - r[0]) / 1e6, r[2] / 1e6, r[3]))
+    return out
+
+
 def counter_table(db: str):
     """(counter names, [(kernel, dispatches, {counter: sum})]) sorted by kernel time."""
     con = sqlite3.connect(db)
@@ -171,6 +195,11 @@ def main(argv):
         print("%-60s %5s " % ("kernel", "calls") + " ".join("%14s" % n[-14:] for n in names))
         for k, n, c in rows:
             print("%-60s %5d " % (k[:60], n) + " ".join("%14.4g" % c.get(x, 0) for x in names))
+        return 0
+    if len(argv) >= 3 and argv[0] == "spans":
+        print("%10s %10s %6s" % ("span ms", "sum ms", "calls"))
+        for span, total, calls in class_spans(argv[1], int(argv[2])):
+            print("%10.3f %10.3f %6d" % (span, total, calls))
         return 0
     if len(argv) >= 3 and argv[0] == "passes":
         print("%-72s %6s %10s %10s" % ("kernel", "calls", "avg ms", "total ms"))
