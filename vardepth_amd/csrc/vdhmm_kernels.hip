@@ -1147,7 +1147,8 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
 
 // 3 waves per SIMD, no spills (158-162 VGPRs with the boundary-row conveyor).
 // At 4 waves the conveyor's registers spill inside the sweep and the kernel
-// measured slower: 57.1 vs 55.6 ms on the 48,592-pair precision-stress batch.
+// measured slower: 57.1 vs 55.6 ms on the 4096-region precision-stress batch
+// (152,114 rescued pairs).
 template <bool ANCH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
                                               int smax, double* __restrict__ wsall, double* __restrict__ out) {
