@@ -9,7 +9,9 @@ workload is 4096 uneven regions (2-32 haplotypes of 150-600 bp, 2-30 or
 The JSON line carries:
 * value      -- whole-job GCUPS = sum(n*m) * steps / wall seconds
 * roofline   -- the FP32 kernels' rate (11 flops per computed cell) over their
-                own event-timed duration, against MI355X's FP32 vector peak;
+                event-timed span (HIP events before the first class kernel
+                and after the last; consecutive classes overlap on two
+                streams), against MI355X's FP32 vector peak;
                 `effective` counts the pairs' n*m instead (haplotype prefix
                 sharing computes fewer cells than that)
 * cpu_baseline -- the reference's CPU path (trie_many + pairhmm_pairs
