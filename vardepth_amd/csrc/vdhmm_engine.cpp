@@ -1,9 +1,11 @@
 // Host side of the MI355X PairHMM engine: the C ABI of include/vdhmm.h.
 //
 // One process-wide engine (device, stream, grow-only device buffers) behind a
-// mutex.  vd_forward_device enqueues the whole batch -- planning, FP32
-// persistent kernel, FP64 rescue -- on one stream with no host round trip, so
-// a caller can capture it in a hipGraph once its buffers exist.
+// mutex.  vd_forward_device enqueues the whole batch -- planning, the FP32
+// persistent class kernels, FP64 rescue -- on the caller's stream with no host
+// round trip (consecutive class kernels alternate with a second stream that
+// forks from and joins back into it), so a caller can capture it in a
+// hipGraph once its buffers exist.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
