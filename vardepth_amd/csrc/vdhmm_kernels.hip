@@ -1064,8 +1064,10 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
         nQ = inQ[l + 32];
       }
     }
+    uint32_t code_next = cl[0];  // codes are filled to tend + 18
     for (int t = 0; t <= tend; ++t) {
-      const uint32_t code = cl[t];
+      const uint32_t code = code_next;
+      code_next = cl[t + 1];
       double pinj, qinj;
       if (first) {
         if (ANCH) {
