@@ -200,6 +200,7 @@ def test_autotune_keeps_results():
     suffix continuation off: it agrees to FP32 rounding, tested on its own)."""
     from vardepth_amd import autotune
     prev_sfx = _lib.lib().vd_suffix_sharing(0)
+    prev_ov = _lib.lib().vd_class_overlap(-1)
     b = workloads.uneven_regions(seed=9, regions=24)
     dev = engine.DeviceBatch(b)
     base = autotune.get()
@@ -229,8 +230,10 @@ def test_autotune_keeps_results():
         autotune.set_tuning(t)
         g = dev.grad(mode=0).cpu().numpy()
         np.testing.assert_allclose(g, g_ref, rtol=1e-12, atol=1e-12)
+    assert rep["class_overlap"] in (0, 1) and set(rep["overlap_ms"]) == {0, 1}
     autotune.set_tuning(base)
     _lib.lib().vd_suffix_sharing(prev_sfx)
+    _lib.lib().vd_class_overlap(prev_ov)
 
 
 def _sharing_cases():

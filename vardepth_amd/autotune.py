@@ -9,7 +9,12 @@ passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
   shared-suffix continuation; anti-diagonals per main-loop iteration (2 or 4,
   pair units) x persistent grid per CU (the occupancy limit, or 2 or 3 blocks
   per CU); before the per-class sweep, the haplotypes per read unit under
-  sharing (4, 8 or 16) by total time;
+  sharing (4, 8 or 16), timed for the whole forward with the class kernels
+  on two streams and on one; after it, the stream layout
+  (``vd_class_overlap``) with the final choices. Two streams fill each
+  kernel's tail, but where one class holds most of the work they share the
+  CUs with it for its whole run, and their time varies more: they are kept
+  only when more than 2% faster;
 * adjoint: register block of 2 steps (two waves per SIMD where that fits) or
   4 steps.
 
@@ -103,20 +108,25 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         dev_batch.forward(mode=mode, precision=precision, stats=True)
 
     run()  # warm: buffers and tables
-    chunk_ms = {}
+    lib = _lib.lib()
+    pair_ms = {}
     base = get()
-    for chunk in CHUNK_CANDIDATES:  # whole-forward time of each read-unit size (sharing on everywhere)
-        t = _lib.VdTuning.from_buffer_copy(base)
-        t.share_chunk = chunk
-        for r in range(1, 17):
-            t.fwd_share[r] = 2
-        set_tuning(t)
-        best = np.inf
-        for _ in range(reps):
-            best = min(best, dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"])
-        chunk_ms[chunk] = best
+    for overlap in (1, 0):  # whole-forward time of each stream layout x read-unit size (sharing on everywhere)
+        lib.vd_class_overlap(overlap)
+        for chunk in CHUNK_CANDIDATES:
+            t = _lib.VdTuning.from_buffer_copy(base)
+            t.share_chunk = chunk
+            for r in range(1, 17):
+                t.fwd_share[r] = 2
+            set_tuning(t)
+            best = np.inf
+            for _ in range(reps):
+                best = min(best, dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"])
+            pair_ms[(overlap, chunk)] = best
+    overlap, chunk = min(pair_ms, key=pair_ms.get)
+    chunk_ms = {c: pair_ms[(overlap, c)] for c in CHUNK_CANDIDATES}
     t = _lib.VdTuning.from_buffer_copy(base)
-    t.share_chunk = min(chunk_ms, key=chunk_ms.get)
+    t.share_chunk = chunk
     set_tuning(t)
     times = _sweep(run, candidates, _apply_fwd, 0, reps)
     default = times[candidates[0]]
@@ -127,8 +137,19 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         _apply_fwd(t, r, c)
     set_tuning(t)
     tuned = sum(times[c][r] for r, c in choice.items())
+    # the stream layout with the final choices: two streams are kept only when
+    # they beat one by more than 2%, since their time varies more from run to
+    # run (it depends on how the two kernels' blocks interleave)
+    overlap_ms = {}
+    for o in (1, 0):
+        lib.vd_class_overlap(o)
+        overlap_ms[o] = min(dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"]
+                            for _ in range(max(reps, 2)))
+    overlap = 1 if overlap_ms[1] < 0.98 * overlap_ms[0] else 0
+    lib.vd_class_overlap(overlap)
     return {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
             "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
+            "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
             "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
 
 
@@ -155,6 +176,7 @@ def save(path, key: str):
     p = Path(path)
     data = json.loads(p.read_text()) if p.exists() else {}
     data[key] = get().as_dict()
+    data[key]["class_overlap"] = int(_lib.lib().vd_class_overlap(-1))
     p.write_text(json.dumps(data, indent=1, sort_keys=True))
 
 
@@ -176,4 +198,6 @@ def load(path, key: str) -> bool:
         for i, v in enumerate(data[key][name]):
             arr[i] = int(v)
     set_tuning(t)
+    if "class_overlap" in data[key]:
+        _lib.lib().vd_class_overlap(int(data[key]["class_overlap"]))
     return True
