@@ -50,6 +50,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = "PairHMM GCUPS (fwd, fwd+bwd) at <=1e-4 log10 err vs FP64; % of FP32 peak"
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_CELL = 11
+CPU_THREAD_CAP = 16   # host threads per GPU on a shared multi-GPU node
+REFERENCE_GAP_EXT = (0.001, 0.1)   # the model trie_many / pairhmm_pairs compute (batch_hmm.cpp:30)
 
 
 def available_cpus() -> int:
@@ -107,8 +109,35 @@ def fp32_cells(batch, mode: int) -> int:
     return total
 
 
-def cpu_baseline(batch, mode: int, seconds: float, threads: int):
-    """Reference CPU chain on a time-bounded sample of regions (largest-first)."""
+def reference_region(ref, batch, k, mode):
+    """The reference CPU chain on region k (trie_vector.py:16-42): trie_many over
+    the reads sorted by length and the sorted haplotypes, NaN values recomputed
+    by pairhmm_pairs; returned in the batch's layout (row-major reads x haplotypes
+    in batch order).  Those two hard-code g = .001, x = .1 (batch_hmm.cpp:30,
+    trie_vector.cpp:219); with other model parameters, or with ref None, the
+    oracle's C restatement of that chain (parametric) is used."""
+    import oracle
+    if ref is None or (batch.gap, batch.ext) != REFERENCE_GAP_EXT:
+        return oracle.regions_cpu(batch.subset([k]), mode=mode, threads=1)[0]
+    reads, quals, haps = batch.region(k)
+    ri = sorted(range(len(reads)), key=lambda i: len(reads[i]))   # trie_vector.py:30
+    hi = sorted(range(len(haps)), key=lambda j: haps[j])          # trie_vector.py:27
+    out, _ = ref.trie_many([reads[i] for i in ri], [quals[i] for i in ri], [haps[j] for j in hi], mode)
+    nan = np.argwhere(np.isnan(out))
+    if len(nan):
+        jobs = [(reads[ri[a]], quals[ri[a]], haps[hi[b]]) for a, b in nan]
+        out[nan[:, 0], nan[:, 1]] = ref.pairhmm_pairs(jobs, mode)   # trie_vector.py:40-41
+    block = np.empty((len(reads), len(haps)))
+    block[np.ix_(ri, hi)] = out
+    return block.ravel()
+
+
+def cpu_baseline(batch, mode: int, seconds: float, threads: int, device_out=None):
+    """Reference CPU chain on a time-bounded random sample of regions.
+
+    With device_out (the headline step's output, batch layout), the sampled
+    regions' reference values are kept and the step's per-pair log10 error
+    against them is reported."""
     import oracle
     from concurrent.futures import ThreadPoolExecutor
     ref = oracle.ref()
@@ -116,25 +145,17 @@ def cpu_baseline(batch, mode: int, seconds: float, threads: int):
     rng = np.random.default_rng(7)
     order = rng.permutation(batch.nregions)
     rl, hl = batch.read_lengths(), batch.hap_lengths()
+    kept = {}
 
-    def one(k):
-        reads, quals, haps = batch.region(k)
-        if ref is not None:
-            ri = sorted(range(len(reads)), key=lambda i: len(reads[i]))   # trie_vector.py:30
-            hs = sorted(haps)                                            # trie_vector.py:27
-            out, _ = ref.trie_many([reads[i] for i in ri], [quals[i] for i in ri], hs, mode)
-            nan = np.argwhere(np.isnan(out))
-            if len(nan):
-                jobs = [(reads[ri[a]], quals[ri[a]], hs[b]) for a, b in nan]
-                out[nan[:, 0], nan[:, 1]] = ref.pairhmm_pairs(jobs, mode)   # trie_vector.py:40-41
-        else:
-            sub = batch.subset([k])
-            oracle.regions_cpu(sub, mode=mode, threads=1)
+    def one(k, keep):
+        vals = reference_region(ref, batch, k, mode)
+        if keep:
+            kept[k] = vals
         a = int(rl[batch.region_reads[k]:batch.region_reads[k + 1]].sum())
         b = int(hl[batch.region_haps[k]:batch.region_haps[k + 1]].sum())
         return a * b
 
-    def run(nthreads, budget):
+    def run(nthreads, budget, keep):
         cells = 0
         done = 0
         t0 = time.perf_counter()
@@ -144,7 +165,7 @@ def cpu_baseline(batch, mode: int, seconds: float, threads: int):
             for _ in range(nthreads * 2):
                 k = next(it, None)
                 if k is not None:
-                    futs.append(ex.submit(one, int(k)))
+                    futs.append(ex.submit(one, int(k), keep))
             while futs:
                 f = futs.pop(0)
                 cells += f.result()
@@ -152,11 +173,11 @@ def cpu_baseline(batch, mode: int, seconds: float, threads: int):
                 if time.perf_counter() - t0 < budget:
                     k = next(it, None)
                     if k is not None:
-                        futs.append(ex.submit(one, int(k)))
+                        futs.append(ex.submit(one, int(k), keep))
         return cells, done, time.perf_counter() - t0
 
-    cells, done, dt = run(threads, seconds)
-    c1, d1, t1 = run(1, max(2.0, seconds / 5))
+    cells, done, dt = run(threads, seconds, device_out is not None)
+    c1, d1, t1 = run(1, max(2.0, seconds / 5), False)
     model = "unknown"
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -165,13 +186,39 @@ def cpu_baseline(batch, mode: int, seconds: float, threads: int):
                 break
     except OSError:
         pass
-    return {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
-            "sample": "%d of %d regions of the same workload (%.3g cells, %.1f s wall), FP64 %s; "
-                      "1 core: %.3f GCUPS over %d regions" % (
-                          done, batch.nregions, cells, dt,
-                          "trie_many+pairhmm_pairs compiled from the reference sources" if kind == "reference"
-                          else "C restatement of trie_many+pairhmm_pairs", c1 / t1 / 1e9, d1),
-            "gcups_1core": c1 / t1 / 1e9, "cpu_model": model}
+    res = {"value": cells / dt / 1e9, "unit": "GCUPS", "cores": threads, "kind": kind,
+           "sample": "%d of %d regions of the same workload (%.3g cells, %.1f s wall), FP64 %s; "
+                     "1 core: %.3f GCUPS over %d regions" % (
+                         done, batch.nregions, cells, dt,
+                         "trie_many+pairhmm_pairs compiled from the reference sources" if kind == "reference"
+                         else "C restatement of trie_many+pairhmm_pairs", c1 / t1 / 1e9, d1),
+           "gcups_1core": c1 / t1 / 1e9, "cpu_model": model,
+           "cores_available": available_cpus(), "cores_online": os.cpu_count(),
+           "core_note": "threads = min(available cores, %d): one GPU's share of the host" % CPU_THREAD_CAP}
+    if device_out is not None and kept:
+        res["accuracy"] = compare_regions(batch, device_out, kept)
+    return res
+
+
+def compare_regions(batch, device_out, expected: dict):
+    """Per-pair |log10 L_device - log10 L_reference| over the regions in
+    `expected` (region -> reference values in batch layout), with the exact
+    agreement of NaN and -inf positions."""
+    oo = batch.out_off()
+    ks = sorted(expected)
+    dev = np.concatenate([device_out[oo[k]:oo[k + 1]] for k in ks])
+    ref = np.concatenate([expected[k] for k in ks])
+    ln10 = math.log(10.0)
+    nan_eq = bool(np.array_equal(np.isnan(dev), np.isnan(ref)))
+    inf_eq = bool(np.array_equal(np.isneginf(dev), np.isneginf(ref)))
+    fin = np.isfinite(ref) & np.isfinite(dev)
+    err = np.abs(dev[fin] - ref[fin]) / ln10
+    return {"regions": len(ks), "pairs": int(len(ref)),
+            "max_log10_err": float(err.max()) if err.size else 0.0,
+            "p99_log10_err": float(np.quantile(err, 0.99)) if err.size else 0.0,
+            "within_1e-4_log10": bool(err.size == 0 or err.max() <= 1e-4),
+            "nan_positions_equal": nan_eq, "neginf_positions_equal": inf_eq,
+            "nonfinite_pairs": int(len(ref) - fin.sum())}
 
 
 def plain_baseline(batch, mode: int, pairs: int):
@@ -296,11 +343,15 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
     (launch choices tuned on the sample itself unless tune is False)."""
     import torch
     import oracle
+    from concurrent.futures import ThreadPoolExecutor
     from vardepth_amd import engine, workloads
+    ref = oracle.ref()
     res = {}
     for depth in depths:
         t0 = time.perf_counter()
-        cs = workloads.chromosome_sample(mb=mb, depth=depth,
+        # the region call's CPU path (pair_scores_many -> cpu_batch -> trie_many)
+        # computes at g = .001, x = .1, so the sample is scored at those values
+        cs = workloads.chromosome_sample(mb=mb, depth=depth, gap=REFERENCE_GAP_EXT[0], ext=REFERENCE_GAP_EXT[1],
                                          progress=lambda m: print(m, file=sys.stderr, flush=True))
         gen_s = time.perf_counter() - t0
         b = cs.batch
@@ -348,7 +399,11 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
             soff = engine.gl_offsets(snh)
             smq = np.concatenate([cs.mq[b.region_reads[k]:b.region_reads[k + 1]] for k in ks])
             t1 = time.perf_counter()
-            sll, _ = oracle.regions_cpu(sub, mode=mode, threads=threads)
+            if ref is not None:
+                with ThreadPoolExecutor(max_workers=threads) as ex:
+                    sll = np.concatenate(list(ex.map(lambda k: reference_region(ref, b, int(k), mode), ks)))
+            else:
+                sll, _ = oracle.regions_cpu(sub, mode=mode, threads=threads)
             sgl, _ = oracle.genotype_regions(sub.region_reads, snh, sub.out_off()[:-1], sll, smq, soff,
                                              threads=threads)
             cpu_t += time.perf_counter() - t1
@@ -368,8 +423,9 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
                     agree += int(np.argmax(sgl[soff[i]:soff[i + 1]]) == calls[k])
         cpu_gcups = cpu_cells / cpu_t / 1e9 if cpu_t > 0 else None
         res["%gx" % depth] = {
-            "workload": "chromosome sample %.3g Mb at %gx (seed 41005): %d regions, %d haplotypes (max %d), "
-                        "%d reads in regions" % (mb, depth, b.nregions, b.nhaps, int(nh.max()), b.nreads),
+            "workload": "chromosome sample %.3g Mb at %gx (seed 41005, g=%.3g, x=%.3g): %d regions, %d haplotypes "
+                        "(max %d), %d reads in regions" % (mb, depth, b.gap, b.ext, b.nregions, b.nhaps,
+                                                           int(nh.max()), b.nreads),
             "regions": b.nregions, "pairs": b.npairs, "genotype_outputs": int(gl_off[-1]), "cells": cells,
             "generate_s": round(gen_s, 1),
             "end_to_end_ms": round(wall * 1e3, 3), "end_to_end_gcups": round(cells / wall / 1e9, 2),
@@ -382,10 +438,15 @@ def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float,
                                  "frac": round(mix_tflops / 78.6, 4)},
             "truth_concordance": round(float(np.mean(calls[var] == cs.true_pair[var])), 4) if var.any() else None,
             "cpu": {"gcups": round(cpu_gcups, 3) if cpu_gcups else None, "threads": threads,
-                    "kind": "port", "sample_regions": int(done), "sample_cells": int(cpu_cells),
-                    "seconds": round(cpu_t, 2),
+                    "cores_available": available_cpus(),
+                    "kind": "reference" if ref is not None and (b.gap, b.ext) == REFERENCE_GAP_EXT else "port",
+                    "sample_regions": int(done),
+                    "sample_cells": int(cpu_cells), "seconds": round(cpu_t, 2),
                     "projected_wall_s": round(cells / (cpu_gcups * 1e9), 1) if cpu_gcups else None,
-                    "what": "oracle regions_cpu (C restatement of trie_many + pairhmm_pairs) + genotype chain"},
+                    "what": ("trie_many + pairhmm_pairs compiled from the reference sources, one region per "
+                             "thread" if ref is not None else "oracle regions_cpu (C restatement of trie_many + "
+                             "pairhmm_pairs)") + ", then the oracle genotype chain (C restatement of "
+                             "molecule_rows / mixture_two / mixture_general)"},
             "speedup_vs_cpu": round(cells / wall / 1e9 / cpu_gcups, 1) if cpu_gcups else None,
             "gl_max_abs_diff_vs_cpu_chain": maxdiff,
             "genotype_stage_max_rel_diff": stage_diff,
@@ -418,8 +479,8 @@ def main():
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
-    ap.add_argument("--geno-mb", type=float, default=6.0,
-                    help="chromosome-sample size for the genotype leg, Mb (0 = skip; 60 = the full sample)")
+    ap.add_argument("--geno-mb", type=float, default=60.0,
+                    help="chromosome-sample size for the genotype leg, Mb (0 = skip; default: the full 60 Mb sample)")
     ap.add_argument("--geno-depths", default="30,2", help="depth profiles of the genotype leg")
     ap.add_argument("--geno-cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only-genotype", action="store_true", help="run only the genotype leg")
@@ -442,7 +503,7 @@ def main():
             del src
         print(json.dumps({"genotype": genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")],
                                                      args.mode, max(1, min(args.steps, 3)), args.geno_cpu_seconds,
-                                                     min(available_cpus(), 16), not args.no_tune)}), flush=True)
+                                                     min(available_cpus(), CPU_THREAD_CAP), not args.no_tune)}), flush=True)
         return
 
     if args.no_forward:
@@ -485,6 +546,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     gcups = cells * args.steps / wall / 1e9
+    headline_out = dev.result()   # the last timed step's likelihoods, checked against the CPU reference below
     fwd32 = float(np.mean([s["fwd32_ms"] for s in steps]))
     last = steps[-1]
     # the FP32 kernels' hardware rate counts the cells they computed (fewer than
@@ -530,7 +592,7 @@ def main():
     genotype = None
     if args.geno_mb > 0:
         genotype = genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")], args.mode,
-                                  max(1, min(args.steps, 3)), args.geno_cpu_seconds, min(available_cpus(), 16),
+                                  max(1, min(args.steps, 3)), args.geno_cpu_seconds, min(available_cpus(), CPU_THREAD_CAP),
                                   not args.no_tune)
 
     if args.copy_calib:
@@ -542,8 +604,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(available_cpus(), 16)
-        cpu = cpu_baseline(batch, args.mode, args.cpu_seconds, threads)
+        threads = min(available_cpus(), CPU_THREAD_CAP)
+        cpu = cpu_baseline(batch, args.mode, args.cpu_seconds, threads, device_out=headline_out)
 
     line = {
         "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1,

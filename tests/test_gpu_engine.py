@@ -376,3 +376,19 @@ def test_calls_on_different_streams_do_not_share_workspaces_concurrently():
         # FP64-rescued pairs may move in the last bit with the rescue list's order
         np.testing.assert_allclose(a.result(), ref_a, rtol=1e-12, atol=0)
         np.testing.assert_allclose(b.result(), ref_b, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_chromosome_sample_forward_vs_oracle(mode):
+    """The forward over a 0.3 Mb slice of the chromosome-scale sample at 30x
+    (prefix sharing and the shared-suffix continuation on, as by default)
+    against the reference CPU chain restated in the oracle (trie, dense
+    fallback), pair by pair; NaN and -inf positions must agree exactly."""
+    b = workloads.chromosome_sample(mb=0.3, depth=30.0).batch
+    out, st = engine.forward(b, mode=mode, precision="mixed", stats=True)
+    expect, dense = oracle.regions_cpu(b, mode=mode)
+    np.testing.assert_array_equal(np.isnan(out), np.isnan(expect))
+    err = close(out, expect, TOL_MIXED)
+    assert st["fp32_cells_swept"] < st["cells"]  # sharing took part
+    print("chromosome 0.3 Mb mode %d: %d regions, %d pairs, max |ln err| %.3g, flagged %d, swept %.3g of %.3g" % (
+        mode, b.nregions, st["pairs"], err, st["flagged"], st["fp32_cells_swept"], st["cells"]))

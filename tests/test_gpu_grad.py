@@ -1,11 +1,14 @@
 """Adjoint kernels on the MI355X: gradients of sum w ln L against FP64 references.
 
 References: the PyTorch float64 autograd restatement (oracle/torch_ref.py,
-itself pinned to the reference's golden values) and central finite
-differences of the C oracle with a relative step h = 1e-3.  Tolerance: 1e-3
+pinned by tests/test_oracle_torchref.py to the reference's golden values, to
+exact rational derivatives of the brute-force path sum and to finite
+differences) and central finite differences of the C oracle with a relative
+step h = 1e-3.  Tolerance: 1e-3
 relative to the largest component of each gradient group (d/dgap, d/dext,
 d/derr), the accuracy contract of the gradient path; ln L within 1e-4 log10.
 """
+import functools
 import math
 
 import numpy as np
@@ -89,16 +92,17 @@ def G_SHAPE():
     return fx.load()["region_shape"]
 
 
-def test_grad_vs_finite_differences():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_grad_vs_finite_differences(mode):
     """Central differences (relative step 1e-3) of the C oracle's sum of ln L."""
     jobs = fx.pairs("region")[:48:3]
     gap, ext = 0.001, 0.1
     err = oracle.error_table()[:64].copy()
-    out, g = engine.grad(batch_of(jobs, gap, ext), mode=0, err=err)
+    out, g = engine.grad(batch_of(jobs, gap, ext), mode=mode, err=err)
 
     def F(gp, xp, ep):
         e256 = np.array([ep[min(q, 63)] for q in range(256)])
-        return sum(oracle.dense(s, q, h, 0, gp, xp, err=e256) for s, q, h in jobs)
+        return sum(oracle.dense(s, q, h, mode, gp, xp, err=e256) for s, q, h in jobs)
 
     h = 1e-3
     fd_g = (F(gap * (1 + h), ext, err) - F(gap * (1 - h), ext, err)) / (2 * h * gap)
@@ -208,4 +212,107 @@ def test_adam_recovers_parameters():
     le = lambda e: np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used])))
     assert le(last["err"]) < 0.5 * le(first["err"]), (le(first["err"]), le(last["err"]))
     print("gap %.4g -> %.4g, ext %.3g -> %.3g, log10 err %.3f -> %.3f" % (
+        first["gap"], last["gap"], first["ext"], last["ext"], le(first["err"]), le(last["err"])))
+
+
+# --- the differentiation workload: 1024 HMM-sampled regions, 2.05 M pairs ---------------------
+
+@functools.lru_cache(maxsize=1)
+def hmm_regions_1024():
+    return workloads.hmm_sampled_regions(seed=41004, regions=1024)
+
+
+def _slice(b, regions, reads, haps):
+    picked = []
+    for k in regions:
+        rs, qs, hs = b.region(k)
+        picked.append((rs[:reads], qs[:reads], hs[:haps]))
+    jobs = [(s.encode(), q, h.encode()) for rs, qs, hs in picked for s, q in zip(rs, qs) for h in hs]
+    return workloads.from_regions(picked, b.gap, b.ext), jobs
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_hmm_regions_slice_vs_autograd_and_fd(mode):
+    """Reads of three of the 1024 regions (first, middle, last) against their
+    first haplotypes: gradients against the float64 autograd restatement and
+    against central differences of the C oracle."""
+    b = hmm_regions_1024()
+    sl, jobs = _slice(b, (0, 511, 1023), 4, 3)
+    err = oracle.error_table()[:64].copy()
+    out, g, st = engine.grad(sl, mode=mode, err=err, stats=True)
+    v, dg, dx, de = torch_ref.value_and_grad(jobs, mode, b.gap, b.ext, err)
+    np.testing.assert_allclose(out, v, rtol=0, atol=TOL_MIXED)
+    check_grad(g, dg, dx, de)
+
+    def F(gp, xp, ep):
+        e256 = np.array([ep[min(q, 63)] for q in range(256)])
+        return sum(oracle.dense(s, q, h, mode, gp, xp, err=e256) for s, q, h in jobs)
+
+    h = 1e-3
+    fd_g = (F(b.gap * (1 + h), b.ext, err) - F(b.gap * (1 - h), b.ext, err)) / (2 * h * b.gap)
+    fd_x = (F(b.gap, b.ext * (1 + h), err) - F(b.gap, b.ext * (1 - h), err)) / (2 * h * b.ext)
+    assert abs(g[0] - fd_g) <= RTOL * abs(fd_g), (g[0], fd_g)
+    assert abs(g[1] - fd_x) <= RTOL * abs(fd_x), (g[1], fd_x)
+    for qb in (10, 25, 40):
+        ep, em = err.copy(), err.copy()
+        ep[qb] *= 1 + h
+        em[qb] *= 1 - h
+        fd = (F(b.gap, b.ext, ep) - F(b.gap, b.ext, em)) / (2 * h * err[qb])
+        assert abs(g[2 + qb] - fd) <= RTOL * np.max(np.abs(de)), (qb, g[2 + qb], fd)
+    print("mode %d: %d pairs, %d flagged, dg %.6g dx %.6g" % (mode, sl.npairs, st["flagged"], g[0], g[1]))
+
+
+def test_hmm_regions_full_batch_truth_weights():
+    """The gradient over all 2.05 M pairs with weight 1 on each read's true
+    haplotype and 0 elsewhere equals the gradient of the true-pair batch, and
+    the ln L of the true pairs agree between the two layouts."""
+    b = hmm_regions_1024()
+    tp = workloads.true_pairs(b)
+    oo = b.out_off()
+    w = np.zeros(b.npairs)
+    idx = []
+    for k in range(b.nregions):
+        r0, r1, h0 = b.region_reads[k], b.region_reads[k + 1], b.region_haps[k]
+        H = b.region_haps[k + 1] - h0
+        local = b.read_true_hap[r0:r1] - h0
+        idx.append(oo[k] + np.arange(r1 - r0) * H + local)
+    idx = np.concatenate(idx)
+    w[idx] = 1.0
+    out_full, g_full, st = engine.grad(b, weights=w, stats=True)
+    out_true, g_true = engine.grad(tp)
+    assert st["pairs"] == 2048000 and st["fp64_bad"] == 0
+    assert np.isfinite(out_full).all()
+    # true pairs, in the true-pair batch's order (region, haplotype, read)
+    order = []
+    for k in range(b.nregions):
+        r0, r1, h0 = b.region_reads[k], b.region_reads[k + 1], b.region_haps[k]
+        H = b.region_haps[k + 1] - h0
+        local = b.read_true_hap[r0:r1] - h0
+        for h in range(H):
+            sel = np.flatnonzero(local == h)
+            order.append(oo[k] + sel * H + h)
+    np.testing.assert_allclose(out_full[np.concatenate(order)], out_true, rtol=0, atol=TOL_MIXED)
+    scale = np.max(np.abs(g_true[2:]))
+    assert abs(g_full[0] - g_true[0]) <= 1e-4 * abs(g_true[0])
+    assert abs(g_full[1] - g_true[1]) <= 1e-4 * abs(g_true[1])
+    assert np.max(np.abs(g_full[2:] - g_true[2:])) <= 1e-4 * scale
+
+
+def test_adam_recovers_parameters_1024_regions():
+    """200 Adam steps on the true-haplotype loss of all 1024 regions
+    (102,400 reads) from the perturbed start."""
+    from vardepth_amd import fit
+    true_gap, true_ext = 0.01, 0.3
+    err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
+    dev = engine.DeviceBatch(workloads.true_pairs(hmm_regions_1024()))
+    gap0, ext0, err0 = fit.perturbed_start(41004, true_gap, true_ext, err_true)
+    _, hist = fit.fit(dev, gap0, ext0, err0, steps=200, lr=0.05)
+    first, last = hist[0], hist[-1]
+    assert last["loss"] < first["loss"]
+    assert abs(last["gap"] - true_gap) < 0.3 * abs(first["gap"] - true_gap), (first["gap"], last["gap"])
+    assert abs(last["ext"] - true_ext) < 0.3 * abs(first["ext"] - true_ext), (first["ext"], last["ext"])
+    used = np.arange(10, 41)
+    le = lambda e: np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used])))
+    assert le(last["err"]) < 0.5 * le(first["err"]), (le(first["err"]), le(last["err"]))
+    print("1024 regions: gap %.4g -> %.4g, ext %.3g -> %.3g, log10 err %.3f -> %.3f" % (
         first["gap"], last["gap"], first["ext"], last["ext"], le(first["err"]), le(last["err"])))
