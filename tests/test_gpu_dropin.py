@@ -13,6 +13,8 @@ import math
 import numpy as np
 import pytest
 
+import oracle
+
 from tests.golden import fixtures as fx
 from vardepth_amd import accelerator, batch, caller, native, trie_vector, whole_pairhmm
 
@@ -189,7 +191,7 @@ def test_accelerator_score_pairs(monkeypatch, precision, tol):
     jobs = _str_jobs("grid")
     close(accelerator.score_pairs(jobs), G["grid_dense_m0"], tol)
     close(accelerator.score_pairs(jobs, anchored=True), G["grid_dense_m1"], tol)
-    last = accelerator._ENGINE.last
+    last = accelerator._engine().last
     assert last["backend"] == "hip" and last["pairs"] == len(jobs) and last["precision"] == precision
     assert last["dp_cells"] == sum(len(s) * len(h) for s, q, h in jobs)
 
@@ -260,3 +262,42 @@ def test_prepared_wide():
     jobs = _str_jobs("grid")
     got = [whole_pairhmm.Prepared([h])(s, q)[0] for s, q, h in jobs]
     close(got, G["grid_wide"], TOL64)
+
+
+def test_cpu_pairs_and_simd_pairs_fp64():
+    """The reference's exact scorers (accelerator.py:42-62): anchored ->
+    anchored_pairhmm's model, else wide read_haplotype's; the SIMD form groups
+    haplotypes by read.  Both FP64 on the device against the reference's own
+    scalar outputs; empty reads or haplotypes give -inf (test_accelerator.py:33-40)."""
+    jobs = _str_jobs("grid")
+    close(accelerator.cpu_pairs(jobs), G["grid_wide"], TOL64)
+    close(accelerator.cpu_pairs(jobs, anchored=True), G["grid_anch"], TOL64)
+    close(accelerator.cpu_simd_pairs(jobs), G["grid_wide"], TOL64)
+    # shuffled pairs with shared reads and repeated qualities keep their order
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(len(jobs))
+    shuffled = [jobs[i] for i in perm] + [("", b"", "A"), ("A", b"\x1e", "")]
+    out = accelerator.cpu_simd_pairs(shuffled)
+    close(out[:-2], G["grid_wide"][perm], TOL64)
+    assert np.isneginf(out[-2:]).all()
+    assert np.isneginf(accelerator.cpu_pairs([("", [], "A"), ("A", [30], "")])).all()
+    assert accelerator.cpu_simd_pairs([]).shape == (0,)
+    # the reference test's cases: same read at two qualities, one read against several haplotypes
+    pairs = [("ACGT" * 8, [30] * 32, "ACGT" * 9), ("ACGT" * 8, [3] * 32, "ACGT" * 9),
+             ("GATTACA", [20] * 7, "GGGATTACAT"), ("ACGT" * 8, [30] * 32, "TCGT" * 9), ("", [], "A"), ("A", [30], "")]
+    np.testing.assert_allclose(accelerator.cpu_simd_pairs(pairs), accelerator.cpu_pairs(pairs), atol=1e-11, rtol=0)
+    expect = [oracle.dense(s.encode(), bytes(q), h.encode(), 2) for s, q, h in pairs]
+    close(accelerator.cpu_pairs(pairs), expect, TOL64)
+
+
+def test_score_many_matches_prepared():
+    """score_many equals Prepared(haps)(read, q) per read (test_accelerator.py:21-27)."""
+    reads = ['ACGT' * 8, 'A' * 30, 'GATTACA']
+    quals = [[30] * len(x) for x in reads]
+    haps = ['TT' + r + 'GG' for r in reads] + ['ACGT' * 10]
+    expected = np.array([whole_pairhmm.Prepared(haps, mode=0)(r, q) for r, q in zip(reads, quals)])
+    for sel in ("cpu", "hip"):
+        np.testing.assert_allclose(accelerator.score_many(reads, quals, haps, selection=sel), expected,
+                                   atol=1e-9, rtol=0)
+    assert accelerator.score_many([], [], haps, selection='cpu').shape == (0, 4)
+    assert accelerator.score_many(reads, quals, [], selection='cpu').shape == (3, 0)
