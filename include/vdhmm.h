@@ -131,7 +131,17 @@ int vd_forward(const vd_batch *batch, const vd_params *params, int mode, int pre
 /* Device pointers in and out (already resident in HBM), launched on `stream`
  * (a hipStream_t, NULL = default stream).  npairs = out_off[nregions];
  * max_hap_len bounds every haplotype length.  If stats is non-NULL the call
- * synchronises the stream and fills it; with stats NULL it only enqueues. */
+ * synchronises the stream and fills it; with stats NULL it only enqueues.
+ *
+ * Graph capture: a call on a capturing stream records its work into the
+ * graph on the workspaces and parameter tables that the eager (uncaptured)
+ * call made right before it left; it allocates and uploads nothing, and
+ * fails (2, vd_last_error) when it would need to -- a larger batch, other
+ * parameters, or buffers that an earlier capture holds.  The graph keeps
+ * those buffers for the life of the process: later calls allocate their own,
+ * so replays reproduce the captured call after any other work.  Replays of
+ * one graph must not overlap each other.  The same holds for
+ * vd_grad_device and the genotype *_device entry points. */
 int vd_forward_device(const vd_batch *dev_batch, const vd_params *params, int mode, int precision,
                       int64_t npairs, int max_hap_len, double *dev_out, void *stream,
                       vd_stats *stats);
@@ -313,6 +323,13 @@ int vd_genotype_guard(const vd_geno_batch *batch, const double *gl, const double
  * Reads of at most 256 rows. */
 int vd_forward_plain(const vd_batch *batch, const vd_params *params, int mode, int64_t max_pairs,
                      double *out, double *gpu_ms);
+
+/* Longest haplotype (bases) a batch may hold: the forward (gradients = 0)
+ * and gradient (gradients = 1) kernels keep each work unit's haplotype codes
+ * in one workgroup's LDS (160 KiB on gfx950, about 20,000 bases).  Calls with
+ * a longer haplotype fail with vd_last_error() set before touching the device.
+ * No counterpart in the reference (its CPU kernels have no such bound). */
+int vd_max_hap_len(int gradients);
 
 /* Message of the last failed call on this process. */
 const char *vd_last_error(void);

@@ -392,3 +392,54 @@ def test_chromosome_sample_forward_vs_oracle(mode):
     assert st["fp32_cells_swept"] < st["cells"]  # sharing took part
     print("chromosome 0.3 Mb mode %d: %d regions, %d pairs, max |ln err| %.3g, flagged %d, swept %.3g of %.3g" % (
         mode, b.nregions, st["pairs"], err, st["flagged"], st["fp32_cells_swept"], st["cells"]))
+
+
+def test_graph_keeps_its_workspaces_and_parameters():
+    """A captured forward replays its own parameters and buffers after a larger
+    batch with other parameters grew and rewrote the engine's workspaces; a
+    capture that would need other parameter tables (or larger buffers) than
+    the last eager call left is refused with an error before it enqueues
+    anything."""
+    import torch
+    small = workloads.uneven_regions(seed=5, regions=16)
+    dev = engine.DeviceBatch(small)
+    g = dev.graph(mode=0, precision="mixed")
+    torch.cuda.synchronize()
+    captured = dev.result()
+    # a capture with other parameters than the last eager call would have to
+    # upload tables inside the graph: refused
+    big = engine.DeviceBatch(workloads.uneven_regions(seed=6, regions=96))
+    g2 = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    with pytest.raises(_lib.VdhmmError, match="stream capture"):
+        with torch.cuda.graph(g2, stream=s):
+            big.forward(mode=0, precision="mixed", stream=s, gap=0.02)
+    del g2
+    # an eager call of the larger batch with other parameters allocates its own workspaces and tables
+    big.forward(mode=1, precision="mixed", gap=0.01, ext=0.3)
+    torch.cuda.synchronize()
+    assert big.max_hap >= dev.max_hap and big.npairs > 4 * dev.npairs
+    for rep in range(3):
+        dev.out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(dev.result(), captured, rtol=0, atol=1e-12)
+        big.forward(mode=0, precision="float64", gap=0.02, ext=0.2, err=np.full(64, 0.01))
+        torch.cuda.synchronize()
+    expect, _ = oracle.regions_cpu(small, mode=0)
+    close(captured, expect, TOL_MIXED)
+
+
+def test_haplotype_length_limit():
+    """Batches with a haplotype longer than the kernels hold in LDS are refused
+    with a clear error; one at the limit is scored."""
+    lim = _lib.lib().vd_max_hap_len(0)
+    assert 19000 < lim < 21000
+    rng = np.random.default_rng(3)
+    read = bytes(rng.choice(list(b"ACGT"), 100).astype(np.uint8))
+    qual = bytes([30] * 100)
+    ok = bytes(rng.choice(list(b"ACGT"), lim).astype(np.uint8))
+    out = engine.forward(workloads.from_regions([([read], [qual], [ok])]), mode=0, precision="mixed")
+    expect = oracle.dense(read, qual, ok, 0)
+    assert abs(out[0] - expect) <= TOL_MIXED

@@ -290,3 +290,19 @@ def test_tuning_get_set_validation():
     assert autotune.get().fwd_unroll[10] == 4 and autotune.get().fwd_share[10] == 0
     assert autotune.get().share_chunk == 16
     autotune.set_tuning(t)
+
+
+def test_haplotype_length_limit_checked_before_the_device():
+    """The longest haplotype the kernels hold in LDS (160 KiB per workgroup on
+    gfx950) is known without a device, and a batch above it is refused before
+    any device call."""
+    from vardepth_amd import _lib
+    lim = _lib.lib().vd_max_hap_len(0)
+    glim = _lib.lib().vd_max_hap_len(1)
+    assert 19000 < glim <= lim < 21000
+    b = workloads.from_regions([(["ACGT"], [b"\x1e" * 4], ["A" * (lim + 1)])])
+    with pytest.raises(_lib.VdhmmError, match="exceeds the %d bases" % lim):
+        engine.forward(b)
+    g = workloads.from_regions([(["ACGT"], [b"\x1e" * 4], ["A" * (glim + 1)])])
+    with pytest.raises(_lib.VdhmmError, match="gradient kernels"):
+        engine.grad(g)

@@ -31,26 +31,64 @@ int fail(const std::string& msg) {
   return 2;
 }
 
+// Workspace epochs.  A call captured into a HIP graph bakes the addresses of
+// the engine's workspaces (plan, trie, suffix, FP64, gradient and genotype
+// scratch, counters) and parameter tables into that graph.  The first eager
+// call after one or more captured calls advances the epoch: every buffer that
+// exists then belongs to the graphs captured so far and is never freed or
+// rewritten again, and eager calls allocate buffers of the new epoch (the
+// held ones stay allocated until the process ends).  Calls captured back to
+// back -- several calls in one graph, or graphs captured with no eager call
+// between them -- share one epoch.  A captured call allocates and uploads
+// nothing: it runs on the buffers and parameter tables the last eager call
+// left, and fails with an error when it would need more.
+unsigned g_epoch = 1;            // epoch of the buffers eager calls use
+bool g_epoch_captured = false;   // a captured call used the current epoch
+bool g_capturing = false;        // the current call is being captured
+bool g_capture_miss = false;     // ... and needed a buffer it may not allocate
+std::vector<void*> g_held;       // buffers of earlier epochs (held by graphs)
+
+const char* const CAPTURE_MISS =
+    "vdhmm: a call under stream capture needs workspaces or parameter tables that the last eager call did not "
+    "leave (a larger batch, other parameters, or buffers an earlier capture holds): make the same call eagerly, "
+    "uncaptured, right before capturing it";
+
 #define HIPCHK(expr)                                                                       \
   do {                                                                                     \
     hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) {                                                                \
+      if (g_capture_miss) return fail(CAPTURE_MISS);                                       \
+      return fail(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    }                                                                                      \
   } while (0)
 
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  unsigned epoch = 0;
+  bool current() const { return p && epoch == g_epoch; }
   hipError_t ensure(size_t bytes) {
-    if (bytes <= cap && p) return hipSuccess;
+    if (bytes <= cap && current()) return hipSuccess;
+    if (g_capturing) {
+      g_capture_miss = true;
+      return hipErrorStreamCaptureUnsupported;
+    }
     if (p) {
-      hipError_t e = hipFree(p);
-      if (e != hipSuccess) return e;
+      if (epoch != g_epoch) {
+        g_held.push_back(p);  // a captured graph may replay into it
+      } else {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return e;
+      }
       p = nullptr;
       cap = 0;
     }
     size_t want = std::max<size_t>(bytes + bytes / 4, 256);
     hipError_t e = hipMalloc(&p, want);
-    if (e == hipSuccess) cap = want;
+    if (e == hipSuccess) {
+      cap = want;
+      epoch = g_epoch;
+    }
     return e;
   }
   template <typename T>
@@ -59,9 +97,44 @@ struct DevBuf {
   }
 };
 
+void open_eager_epoch() {
+  if (g_epoch_captured) {
+    ++g_epoch;
+    g_epoch_captured = false;
+  }
+}
+
+// The engine mutex for entry points that never run under capture (host
+// buffers, synchronous): an eager call after captured ones opens a new epoch.
+struct EagerLock {
+  std::lock_guard<std::mutex> lk{g_mu};
+  EagerLock() { open_eager_epoch(); }
+};
+
+// Set for the duration of one device entry point: whether `st` is capturing.
+// An eager call after captured ones opens a new epoch first.
+struct CaptureScope {
+  explicit CaptureScope(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    g_capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    g_capture_miss = false;
+    if (g_capturing)
+      g_epoch_captured = true;
+    else
+      open_eager_epoch();
+  }
+  ~CaptureScope() {
+    g_capturing = false;
+    g_capture_miss = false;
+  }
+  CaptureScope(const CaptureScope&) = delete;
+  CaptureScope& operator=(const CaptureScope&) = delete;
+};
+
 struct Engine {
   int dev = -1;
   int cus = 0;
+  size_t lds_per_block = 0;  // LDS one workgroup may allocate (device property)
   hipStream_t own_stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // staged inputs / outputs of the host entry points
@@ -113,6 +186,7 @@ int engine_init() {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   g.cus = prop.multiProcessorCount;
+  g.lds_per_block = prop.sharedMemPerBlock;
   HIPCHK(hipStreamCreateWithFlags(&g.own_stream, hipStreamNonBlocking));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
   for (int i = 0; i < 17; ++i) HIPCHK(hipEventCreate(&g.ev_cls[i]));
@@ -123,6 +197,36 @@ int engine_init() {
   const char* ov = std::getenv("VARDEPTH_CLASS_OVERLAP");
   if (ov) g.overlap = ov[0] != '0';
   g.dev = dev;
+  return 0;
+}
+
+// The forward and gradient kernels keep each group's haplotype codes
+// (columns -17 .. m+30, 2 bytes each, 4 groups) and the quality tables in LDS,
+// so one workgroup's LDS bounds the longest haplotype of a batch.  gfx950
+// gives a workgroup 160 KiB; the device's own figure replaces it once the
+// engine is initialised.
+constexpr size_t LDS_GFX950 = 163840;
+
+int max_hap_len_for(int gradients) {
+  const size_t limit = g.lds_per_block ? g.lds_per_block : LDS_GFX950;
+  // largest slot count s (a multiple of 8) whose LDS fits, then m = s - 48
+  size_t lo = 0, hi = limit;  // lds(s) grows with s
+  while (lo < hi) {
+    const size_t mid = (lo + hi + 1) / 2;
+    const int s = (int)(mid * 8);
+    const size_t need = gradients ? std::max(vd::lds_grad32(s), vd::lds_grad64(s))
+                                  : std::max(vd::lds_fwd32(s), vd::lds_fwd64(s));
+    if (need <= limit) lo = mid; else hi = mid - 1;
+  }
+  return (int)std::max<int64_t>(0, (int64_t)lo * 8 - 48);
+}
+
+int check_hap_len(int max_hap_len, int gradients) {
+  const int lim = max_hap_len_for(gradients);
+  if (max_hap_len > lim)
+    return fail("vdhmm: a haplotype of " + std::to_string(max_hap_len) + " bases exceeds the " +
+                std::to_string(lim) + " bases the " + (gradients ? "gradient" : "forward") +
+                " kernels hold in one workgroup's LDS; score longer haplotypes in pieces or elsewhere");
   return 0;
 }
 
@@ -159,8 +263,7 @@ struct StreamOrder {
   hipStream_t st;
   bool on = false;
   explicit StreamOrder(hipStream_t s) : st(s) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (!g.ev_last || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    if (!g.ev_last || g_capturing) return;
     on = true;
     if (g.have_last) (void)hipStreamWaitEvent(st, g.ev_last, 0);
   }
@@ -179,20 +282,32 @@ int check_params(const vd_params& p) {
   return 0;
 }
 
+void build_grad_tab(const vd_params& p, vd::GradTab& t);
+
+// The parameter tables of `p` on the device (scaled FP32 / FP64 emission and
+// transition tables, exponents, the gradient's derivative table).  Uploaded
+// only when the parameters or the table buffers changed; under capture the
+// tables of the last eager call must already hold `p`.
 int upload_params(const vd_params& p, hipStream_t st) {
-  if (g.have_params && std::memcmp(&p, &g.params_on_device, sizeof(p)) == 0) return 0;
+  const bool same = g.have_params && std::memcmp(&p, &g.params_on_device, sizeof(p)) == 0;
+  if (same && g.qE.current() && g.tab32.current() && g.tab64.current() && g.gtab.current()) return 0;
+  if (g_capturing) return fail(CAPTURE_MISS);
   static vd::QTab<float> t32;
   static vd::QTab<double> t64;
+  static vd::GradTab gt;
   static int32_t E[64];
   build_tab(p, t32, E);
   build_tab(p, t64, E);
+  build_grad_tab(p, gt);
   HIPCHK(g.qE.ensure(sizeof(E)));
   HIPCHK(g.tab32.ensure(sizeof(t32)));
   HIPCHK(g.tab64.ensure(sizeof(t64)));
+  HIPCHK(g.gtab.ensure(sizeof(gt)));
   HIPCHK(hipMemcpyAsync(g.qE.p, E, sizeof(E), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(g.tab32.p, &t32, sizeof(t32), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(g.tab64.p, &t64, sizeof(t64), hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(g.gtab.p, &gt, sizeof(gt), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the static host tables are rewritten by the next upload
   g.params_on_device = p;
   g.have_params = true;
   return 0;
@@ -248,6 +363,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     return fail("vdhmm: negative batch size");
   if (int rc = check_params(*params)) return rc;
   if (int rc = engine_init()) return rc;
+  if (int rc = check_hap_len(max_hap_len, 0)) return rc;
   StreamOrder order(st);
   if (int rc = upload_params(*params, st)) return rc;
   const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
@@ -437,6 +553,7 @@ int forward_host_locked(const vd_batch* hb, const vd_params* params, int mode, i
   int max_hap = 0;
   uint64_t cells = 0;
   if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  if (int rc = check_hap_len(max_hap, 0)) return rc;
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
   StreamOrder order(st);
@@ -490,12 +607,9 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (int rc = check_params(*params)) return rc;
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
+  if (int rc = check_hap_len(max_hap_len, 1)) return rc;
   StreamOrder order(st);
   if (int rc = upload_params(*params, st)) return rc;
-  static vd::GradTab gt;
-  build_grad_tab(*params, gt);
-  HIPCHK(g.gtab.ensure(sizeof(gt)));
-  HIPCHK(hipMemcpyAsync(g.gtab.p, &gt, sizeof(gt), hipMemcpyHostToDevice, st));
   const int smax = ((max_hap_len + 48) + 7) & ~7;
   if (int rc = plan_buffers(db, npairs)) return rc;
   if (g.gocc_smax != smax) {
@@ -583,6 +697,7 @@ int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, cons
     return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
   if (int rc = check_params(*params)) return rc;
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
+  if (int rc = check_hap_len(max_hap, 1)) return rc;
   if (int rc = engine_init()) return rc;
   hipStream_t st = g.own_stream;
   StreamOrder order(st);
@@ -686,7 +801,8 @@ GenoBufs gg;
 int geno_init(hipStream_t st) {
   if (!gg.ev[0])
     for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&gg.ev[i]));
-  if (!gg.mq_err_ready) {
+  if (!gg.mq_err_ready || !gg.mq_err.current()) {
+    if (g_capturing) return fail(CAPTURE_MISS);
     // molecule_rows.py:4: _MQ_ERROR[mq] = 10**(-min(mq,60)/10)
     static double t[256];
     for (int q = 0; q < 256; ++q) t[q] = std::pow(10., -std::min(q, 60) / 10.);
@@ -994,7 +1110,7 @@ double single(const char* read, const unsigned char* qual, int n, const char* ha
   hb.add_hap(hap, m);
   hb.close_region();
   double v = NAN;
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   if (run_host(hb, gap, ext, mode, VD_PREC_FP64, &v)) {
     std::fprintf(stderr, "%s\n", g_err.c_str());
     return NAN;
@@ -1009,7 +1125,7 @@ void one_read_batch(const char* read, const unsigned char* qual, int n, const ch
   hb.add_read(read, qual, n);
   for (int k = 0; k < count; ++k) hb.add_hap(haps[k], lengths[k]);
   hb.close_region();
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   if (run_host(hb, gap, ext, mode, VD_PREC_FP64, out)) {
     std::fprintf(stderr, "%s\n", g_err.c_str());
     for (int k = 0; k < count; ++k) out[k] = NAN;
@@ -1032,7 +1148,7 @@ void vd_params_default(vd_params* p, double gap, double ext) {
 int vd_forward(const vd_batch* batch, const vd_params* params, int mode, int precision, double* out,
                vd_stats* stats) {
   if (!batch || !params || !out) return fail("vdhmm: null argument");
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return forward_host_locked(batch, params, mode, precision, out, stats);
 }
 
@@ -1040,6 +1156,7 @@ int vd_forward_device(const vd_batch* dev_batch, const vd_params* params, int mo
                       int max_hap_len, double* dev_out, void* stream, vd_stats* stats) {
   if (!dev_batch || !params || !dev_out) return fail("vdhmm: null argument");
   std::lock_guard<std::mutex> lk(g_mu);
+  CaptureScope capture((hipStream_t)stream);
   if (int rc = engine_init()) return rc;
   auto t0 = std::chrono::steady_clock::now();
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -1053,7 +1170,7 @@ int vd_forward_device(const vd_batch* dev_batch, const vd_params* params, int mo
 int vd_grad(const vd_batch* batch, const vd_params* params, int mode, const double* weights, double* out,
             double* grad, vd_stats* stats) {
   if (!batch || !params || !grad) return fail("vdhmm: null argument");
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return grad_host_locked(batch, params, mode, weights, out, grad, stats);
 }
 
@@ -1062,6 +1179,7 @@ int vd_grad_device(const vd_batch* dev_batch, const vd_params* params, int mode,
                    vd_stats* stats) {
   if (!dev_batch || !params || !dev_out || !dev_grad) return fail("vdhmm: null argument");
   std::lock_guard<std::mutex> lk(g_mu);
+  CaptureScope capture((hipStream_t)stream);
   if (int rc = engine_init()) return rc;
   auto t0 = std::chrono::steady_clock::now();
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -1075,30 +1193,31 @@ int vd_grad_device(const vd_batch* dev_batch, const vd_params* params, int mode,
 int vd_forward_plain(const vd_batch* batch, const vd_params* params, int mode, int64_t max_pairs, double* out,
                      double* gpu_ms) {
   if (!batch || !params || !out) return fail("vdhmm: null argument");
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return plain_host_locked(batch, params, mode, max_pairs, out, gpu_ms);
 }
 
 int vd_mixture_general(int rows, int n, const double* lh, double prior, int triangular, double* out) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return mixture_host_locked(rows, n, lh, prior, triangular ? 1 : 0, out);
 }
 
 int vd_mixture_two(int rows, const double* lh, double prior, double* out) {
   if (int rc = check_prior(prior, false)) return rc;  // mixture_cpu.py:14: 0 <= prior < 1
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return mixture_host_locked(rows, 2, lh, prior, 0, out);
 }
 
 int vd_normalized_rows(int rows, int n, const double* ll, const unsigned char* mq, double* values,
                        double* certainty) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return normalized_host_locked(rows, n, ll, mq, values, certainty);
 }
 
 int vd_mixture_regions_device(const vd_geno_batch* batch, const double* lh, const signed char* keep, double prior,
                               int triangular, double* gl, void* stream, vd_geno_stats* stats) {
   std::lock_guard<std::mutex> lk(g_mu);
+  CaptureScope capture((hipStream_t)stream);
   auto t0 = std::chrono::steady_clock::now();
   if (stats) std::memset(stats, 0, sizeof(*stats));
   int rc = mixture_device_locked(batch, lh, (const int8_t*)keep, prior, triangular ? 1 : 0, gl, (hipStream_t)stream,
@@ -1112,6 +1231,7 @@ int vd_genotype_device(const vd_geno_batch* batch, const double* ll, const unsig
                        double prior, double* gl, double* values, double* certainty, void* stream,
                        vd_geno_stats* stats) {
   std::lock_guard<std::mutex> lk(g_mu);
+  CaptureScope capture((hipStream_t)stream);
   auto t0 = std::chrono::steady_clock::now();
   if (stats) std::memset(stats, 0, sizeof(*stats));
   int rc = genotype_device_locked(batch, ll, mq, min_certainty, prior, gl, values, certainty, (hipStream_t)stream,
@@ -1124,8 +1244,14 @@ int vd_genotype_device(const vd_geno_batch* batch, const double* ll, const unsig
 int vd_genotype_guard(const vd_geno_batch* batch, const double* gl, const double* certainty, double min_certainty,
                       double margin_scale, signed char* flags, double* gaps, void* stream, int64_t* nflagged) {
   std::lock_guard<std::mutex> lk(g_mu);
+  CaptureScope capture((hipStream_t)stream);
   return guard_device_locked(batch, gl, certainty, min_certainty, margin_scale, (int8_t*)flags, gaps,
                              (hipStream_t)stream, nflagged);
+}
+
+int vd_max_hap_len(int gradients) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return max_hap_len_for(gradients ? 1 : 0);
 }
 
 int vd_tuning_get(vd_tuning* t) {
@@ -1204,7 +1330,7 @@ int pairhmm_pairs(const char* const* reads, const unsigned char* const* quals, c
     hb.add_hap(haps[p], hn[p]);
     hb.close_region();
   }
-  std::lock_guard<std::mutex> lk(g_mu);
+  EagerLock lk;
   return run_host(hb, 0.001, 0.1, mode, VD_PREC_FP64, output) ? 2 : 0;
 }
 
@@ -1218,7 +1344,7 @@ int trie_many(const char* const* reads, const unsigned char* const* quals, const
   for (int h = 0; h < nh; ++h) hb.add_hap(haps[h], hn[h]);
   hb.close_region();
   {
-    std::lock_guard<std::mutex> lk(g_mu);
+    EagerLock lk;
     if (run_host(hb, 0.001, 0.1, mode, env_precision(), out)) return 2;
   }
   // trie_vector.cpp:277: -inf for empty inputs, NaN when the likelihood is <= 1e-200

@@ -150,8 +150,11 @@ class DeviceBatch:
 
         A first eager forward sizes the device buffers and uploads the
         parameter tables, so the captured call only enqueues kernels (counters
-        are cleared by a kernel, not a memset).  Replaying the returned
-        torch.cuda.CUDAGraph recomputes self.out for the same parameters."""
+        are cleared by a kernel, not a memset).  The graph keeps those buffers
+        and tables: later calls of any size or parameters allocate their own,
+        so replaying the returned torch.cuda.CUDAGraph recomputes self.out for
+        the captured parameters at any later time (replays of one graph must
+        not overlap)."""
         torch = self.torch
         self.forward(mode=mode, precision=precision, err=err, gap=gap, ext=ext)
         torch.cuda.synchronize()
