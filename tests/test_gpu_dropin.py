@@ -301,3 +301,32 @@ def test_score_many_matches_prepared():
                                    atol=1e-9, rtol=0)
     assert accelerator.score_many([], [], haps, selection='cpu').shape == (0, 4)
     assert accelerator.score_many(reads, quals, [], selection='cpu').shape == (3, 0)
+
+
+def test_canonical_batch_equals_scalar_anchored():
+    """cpu_batch.pairs(jobs, mode=1, canonical=True) is bit-identical to the
+    scalar anchored_pairhmm, on repeated calls (test_cpu_catalog.py:22-31);
+    here both run FP64 on the device, the batch as one launch of all jobs."""
+    rng = np.random.default_rng(7713)
+    jobs, expected = [], []
+    lib = native.kernel()
+    for _ in range(31):
+        n = int(rng.integers(12, 140))
+        s = ''.join(rng.choice(list('ACGT'), n))
+        h = s[:n // 2] + 'TA' + s[n // 2:]
+        q = rng.integers(0, 61, n).tolist()
+        jobs.append((s, q, h))
+        expected.append(lib.anchored_pairhmm(s.encode(), (ctypes.c_ubyte * n)(*q), n, h.encode(), len(h), .001, .1))
+    for repeat in range(2):
+        batch._CACHE.clear()
+        np.testing.assert_array_equal(batch.pairs(jobs, mode=1, canonical=True), expected)
+        np.testing.assert_array_equal(batch.pairs(jobs, mode=1, canonical=True), expected)   # from the cache
+    # and the same values in another order and inside a larger batch
+    batch._CACHE.clear()
+    perm = rng.permutation(len(jobs))
+    extra = [(''.join(rng.choice(list('ACGT'), 240)), [30] * 240, ''.join(rng.choice(list('ACGT'), 300)))
+             for _ in range(9)]
+    got = batch.pairs([jobs[i] for i in perm] + extra, mode=1, canonical=True)
+    np.testing.assert_array_equal(got[:len(jobs)], np.asarray(expected)[perm])
+    ref = [oracle.dense(s.encode(), bytes(q), h.encode(), 1) for s, q, h in jobs]
+    close(expected, ref, TOL64)

@@ -990,9 +990,9 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
   const int tmain = gp.m_min + 2;
   fill_codes(b, gp, codes, smax, tend + 18);
   const int np = gp.n + (ANCH ? 1 : 0);
-  int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
-  int ntiles = max(max(lane16(ntile_g, 0), lane16(ntile_g, 1)), max(lane16(ntile_g, 2), lane16(ntile_g, 3)));
-  const int pad = ntiles * GROUP * R - np;
+  const int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
+  const int ntiles = max(max(lane16(ntile_g, 0), lane16(ntile_g, 1)), max(lane16(ntile_g, 2), lane16(ntile_g, 3)));
+  const int pad = ntiles * GROUP * R - np;  // reads are bottom-aligned: pass-through rows on top
   const int64_t rb = b.read_off[gp.read];
   const double gam = tab->gamma, xx = tab->ext;
   // boundary buffers: [buffer][stream][group][slot], slot = column + 17
@@ -1113,10 +1113,15 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     }
     if (!last) {
       // renormalise the boundary row to 2^CEXP64; 2^s is applied as two
-      // factors because s can exceed the FP64 exponent range of one factor
+      // factors because s can exceed the FP64 exponent range of one factor.
+      // A tile above the group's first read row (a unit mate has more tiles)
+      // holds pass-through rows only and hands its injection on exactly,
+      // unscaled, so the pair's real tiles see the values and renormalisations
+      // of a unit of its own: its result depends on the pair alone.
       int e = 0;
       if (bmax > 0.0) frexp(bmax, &e);
-      int s = bmax > 0.0 ? CEXP64 - e : 0;
+      const bool real_rows = tile >= ntiles - ntile_g;
+      int s = (bmax > 0.0 && real_rows) ? CEXP64 - e : 0;
       s = s > 2040 ? 2040 : s;
       const int src = g * GROUP + GROUP - 1;
       const int s1 = s / 2, s2 = s - s1;
