@@ -146,6 +146,25 @@ int vd_forward_device(const vd_batch *dev_batch, const vd_params *params, int mo
                       int64_t npairs, int max_hap_len, double *dev_out, void *stream,
                       vd_stats *stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the forward of vd_forward_device with the same arguments
+ * (no stats), every scratch buffer -- plan, FP64 work list, counters,
+ * parameter tables, FP64 tile rows -- taken from one static device array of
+ * 64 MiB per GPU, held by the library until the process ends.  The call
+ * allocates and frees nothing and copies nothing from the host: it only
+ * enqueues kernels on `stream`.  Limits: 13 bytes per read, 17 per pair and
+ * 204 per region plus one FP64 tile workspace of 128 * (max_hap_len + 48)
+ * bytes must fit in the 64 MiB (about 3.5 M pairs of 150-600 bp haplotypes);
+ * a larger batch is refused (2, vd_last_error) before anything is enqueued.
+ * The forward runs without haplotype prefix sharing and on one stream; its
+ * results equal vd_forward_device's with vd_suffix_sharing off (bit for bit)
+ * and agree to FP32 rounding with it on.  Every call on one GPU shares that
+ * GPU's array: calls whose device work overlaps (other streams, no ordering
+ * by the caller) race. */
+int vd_forward_device_noalloc(const vd_batch *dev_batch, const vd_params *params, int mode, int precision,
+                              int64_t npairs, int max_hap_len, double *dev_out, void *stream);
+
 /* ---- gradients (no counterpart in the reference) ------------------------ */
 
 /* Gradient of sum_p w_p ln L_p over every pair of the batch with respect to

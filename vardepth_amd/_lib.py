@@ -32,7 +32,7 @@ SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch
            "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_mixture_general", "vd_mixture_two",
            "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_genotype_guard", "vd_prefix_sharing",
            "vd_suffix_sharing", "vd_class_overlap", "vd_max_hap_len",
-           "vd_last_error", "vd_device")
+           "vd_forward_device_noalloc", "vd_last_error", "vd_device")
 
 
 class VdParams(ctypes.Structure):
@@ -173,6 +173,10 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.POINTER(ctypes.c_int64)]
     L.vd_genotype_guard.restype = ctypes.c_int
+    L.vd_forward_device_noalloc.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+    L.vd_forward_device_noalloc.restype = ctypes.c_int
     L.vd_max_hap_len.argtypes = [ctypes.c_int]
     L.vd_max_hap_len.restype = ctypes.c_int
     L.vd_last_error.argtypes = []

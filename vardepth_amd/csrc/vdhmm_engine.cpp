@@ -485,6 +485,97 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   return 0;
 }
 
+// The forward of vd_forward_device with every scratch buffer carved from the
+// library's static device array (vdhmm_noalloc.hip): plan scratch, FP64 work
+// list, counters, parameter tables and the FP64 tile rows.  No allocation, no
+// host-to-device copy (the tables go as kernel arguments), no prefix sharing
+// (its wavefront workspace scales with the grid) and one stream; the FP64
+// rescue grid shrinks to the scratch left after the plan.
+int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode, int precision, int64_t npairs,
+                           int max_hap_len, double* dout, hipStream_t st) {
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (precision != VD_PREC_MIXED && precision != VD_PREC_FP64) return fail("vdhmm: bad precision");
+  if (db->nregions < 0 || db->nreads < 0 || db->nhaps < 0 || npairs < 0 || max_hap_len < 0)
+    return fail("vdhmm: negative batch size");
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  if (int rc = check_hap_len(max_hap_len, 0)) return rc;
+  void* base = nullptr;
+  size_t cap = 0;
+  HIPCHK(vd::noalloc_scratch(g.dev, &base, &cap));
+  const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
+  const int force64 = precision == VD_PREC_FP64 ? 1 : 0;
+  const int smax = ((max_hap_len + 48) + 7) & ~7;
+  const int64_t nreads = std::max<int64_t>(db->nreads, 1), nreg = std::max<int64_t>(db->nregions, 1);
+  const int64_t nb = db->nregions * vd::NCLS;
+  const int64_t max_units = npairs / vd::PAIRS_PER_UNIT + nb + 1;
+  // layout: 256-byte aligned slices, in order
+  size_t off = 0;
+  auto slice = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) & ~size_t(255);
+    return at;
+  };
+  const size_t o_qE = slice(64 * sizeof(int32_t)), o_t32 = slice(sizeof(vd::QTab<float>)),
+               o_t64 = slice(sizeof(vd::QTab<double>)), o_cls = slice(nreads), o_exp = slice(nreads * 4),
+               o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8), o_ucnt = slice((nb + 1) * 8),
+               o_ubkt = slice(max_units * 4), o_f64 = slice(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)),
+               o_ctr = slice(vd::CTR_COUNT * 8);
+  const size_t ws_block = (size_t)4 * vd::PAIRS_PER_UNIT * smax * sizeof(double);
+  if (off + ws_block > cap)
+    return fail("vdhmm: this batch needs " + std::to_string(off + ws_block) +
+                " bytes of scratch (13 per read, 17 per pair, 204 per region and one FP64 tile workspace of 128 per "
+                "haplotype slot); the no-allocation forward holds " + std::to_string(cap));
+  if (g.occ_smax != smax) {
+    for (int a = 0; a < 2; ++a) {
+      for (int r = 1; r <= vd::MAX_R32; ++r) {
+        HIPCHK(vd::occupancy(a, r, 2, smax, &g.occ32[a][r][0]));
+        HIPCHK(vd::occupancy(a, r, 4, smax, &g.occ32[a][r][1]));
+      }
+      HIPCHK(vd::occupancy(a, 0, 2, smax, &g.occ64[a]));
+    }
+    g.occ_smax = smax;
+  }
+  tune_defaults();
+  int blocks32[17] = {0}, unroll32[17] = {0};
+  for (int r = 1; r <= vd::MAX_R32; ++r) {
+    unroll32[r] = g.tune.fwd_unroll[r] == 4 ? 4 : 2;
+    const int occ = std::max(1, g.occ32[anchored][r][unroll32[r] == 4 ? 1 : 0]);
+    const int want = g.tune.fwd_blocks_per_cu[r];
+    blocks32[r] = g.cus * (want > 0 ? std::min(want, occ) : occ);
+  }
+  const int64_t fit = (int64_t)((cap - off) / ws_block);
+  const int blocks64 = (int)std::max<int64_t>(
+      1, std::min<int64_t>(fit, (int64_t)g.cus * std::max(1, std::min(g.occ64[anchored], 16))));
+  const size_t o_ws = slice((size_t)blocks64 * ws_block);
+  char* B = static_cast<char*>(base);
+  static vd::QTab<float> t32;
+  static vd::QTab<double> t64;
+  static int32_t E[64];
+  build_tab(*params, t32, E);
+  build_tab(*params, t64, E);
+  auto* dE = reinterpret_cast<int32_t*>(B + o_qE);
+  auto* d32 = reinterpret_cast<vd::QTab<float>*>(B + o_t32);
+  auto* d64 = reinterpret_cast<vd::QTab<double>*>(B + o_t64);
+  HIPCHK(vd::launch_put_tables(t32, t64, E, d32, d64, dE, st));
+  vd::PlanDev p{};
+  p.read_cls = reinterpret_cast<int8_t*>(B + o_cls);
+  p.read_exp = reinterpret_cast<int32_t*>(B + o_exp);
+  p.class_off = reinterpret_cast<int32_t*>(B + o_coff);
+  p.read_perm = reinterpret_cast<int64_t*>(B + o_perm);
+  p.unit_cnt = reinterpret_cast<int64_t*>(B + o_ucnt);
+  p.unit_bucket = reinterpret_cast<int32_t*>(B + o_ubkt);
+  p.f64_list = reinterpret_cast<vd::PairRef*>(B + o_f64);
+  p.ctr = reinterpret_cast<unsigned long long*>(B + o_ctr);
+  p.f64_cap = std::max<int64_t>(npairs, 1);
+  p.class_major = 1;
+  p.trie = 0;
+  p.trie_chunk = g.tune.share_chunk;
+  HIPCHK(vd::launch_pipeline(batch_dev(db), p, dE, d32, d64, anchored, force64, smax, blocks32, blocks32, unroll32,
+                             blocks64, reinterpret_cast<double*>(B + o_ws), dout, st, nullptr, nullptr, nullptr));
+  return 0;
+}
+
 int validate_host_batch(const vd_batch* b, int64_t* npairs, int* max_hap, uint64_t* cells) {
   if (b->nregions < 0 || b->nreads < 0 || b->nhaps < 0) return fail("vdhmm: negative batch size");
   if (b->read_off[0] != 0 || b->hap_off[0] != 0 || b->region_reads[0] != 0 || b->region_haps[0] != 0 ||
@@ -1165,6 +1256,14 @@ int vd_forward_device(const vd_batch* dev_batch, const vd_params* params, int mo
   if (stats && !rc)
     stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+int vd_forward_device_noalloc(const vd_batch* dev_batch, const vd_params* params, int mode, int precision,
+                              int64_t npairs, int max_hap_len, double* dev_out, void* stream) {
+  if (!dev_batch || !params || !dev_out) return fail("vdhmm: null argument");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return forward_noalloc_locked(dev_batch, params, mode, precision, npairs, max_hap_len, dev_out,
+                                (hipStream_t)stream);
 }
 
 int vd_grad(const vd_batch* batch, const vd_params* params, int mode, const double* weights, double* out,

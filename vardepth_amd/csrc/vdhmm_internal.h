@@ -198,6 +198,11 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
                        double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
+// scratch of the no-allocation forward (vdhmm_noalloc.hip): device address of
+// the library's static array on device `dev`, and its size
+hipError_t noalloc_scratch(int dev, void** ptr, size_t* bytes);
+hipError_t launch_put_tables(const QTab<float>& t32, const QTab<double>& t64, const int32_t* E, QTab<float>* d32,
+                             QTab<double>* d64, int32_t* dE, hipStream_t st);
 hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, double min_cert, double margin_scale,
                         int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st);
 hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq, const double* mq_err,
