@@ -338,6 +338,71 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "autotune": {"forward": tune_f, "adjoint": tune_g}}
 
 
+def scalar_bench(pairs: int = 2000, calls: int = 400):
+    """Per-call latency of the scalar drop-ins on the short anchored windows the
+    reference scores one read at a time (indel_refine.py:160-163,
+    partial_anchor.py:89-97: a cropped read against an allele with 3 bp of
+    flank): anchored_pairhmm and wide read_haplotype of this library (one
+    device call per pair) against the reference's scalar C++ built into
+    oracle/_ref (one core), and the batched drop-in (cpu_batch.pairs(mode=1,
+    canonical=True): all pairs in one device call) for the same pairs."""
+    import ctypes
+    import oracle
+    from vardepth_amd import batch, native
+    rng = np.random.default_rng(13)
+    jobs = []
+    for _ in range(pairs):
+        n = int(rng.integers(20, 81))
+        s = bytes(rng.choice(list(b"ACGT"), n).astype(np.uint8))
+        cut = int(rng.integers(3, n - 3))
+        h = s[:cut] + bytes(rng.choice(list(b"ACGT"), int(rng.integers(0, 6))).astype(np.uint8)) + s[cut + int(rng.integers(0, 3)):]
+        q = bytes(rng.integers(10, 41, n).astype(np.uint8))
+        jobs.append((s, q, h))
+    qbufs = [(ctypes.c_ubyte * len(q)).from_buffer_copy(q) for _, q, _ in jobs]
+    out = {"workload": "%d cropped reads of 20-80 bp against their allele windows (0-5 bp indel), Phred 10-40"
+                       % pairs}
+    fns = {"anchored_pairhmm": (native.kernel().anchored_pairhmm, "anchored"),
+           "read_haplotype": (native.wide_library().read_haplotype, "wide")}
+    ref = oracle.ref()
+    for name, (fn, refname) in fns.items():
+        fn(jobs[0][0], qbufs[0], len(jobs[0][0]), jobs[0][2], len(jobs[0][2]), .001, .1)   # warm
+        t0 = time.perf_counter()
+        for i in range(calls):
+            s, _, h = jobs[i % pairs]
+            fn(s, qbufs[i % pairs], len(s), h, len(h), .001, .1)
+        gpu_us = (time.perf_counter() - t0) / calls * 1e6
+        rec = {"gpu_us_per_call": round(gpu_us, 2)}
+        if ref is not None:
+            rf = {"anchored": ref.anchored.anchored_pairhmm, "wide": ref.wide.read_haplotype}[refname]
+            t0 = time.perf_counter()
+            for i in range(pairs):
+                s, _, h = jobs[i]
+                rf(s, qbufs[i], len(s), h, len(h), .001, .1)
+            rec["cpu_reference_us_per_call"] = round((time.perf_counter() - t0) / pairs * 1e6, 3)
+        out[name] = rec
+    strjobs = [(s.decode(), q, h.decode()) for s, q, h in jobs]
+
+    def batched(k):
+        batch._CACHE.clear()
+        t0 = time.perf_counter()
+        batch.pairs(strjobs[:k], mode=1, canonical=True)
+        return time.perf_counter() - t0
+
+    batched(16)   # warm
+    t_small = min(batched(16) for _ in range(5))
+    t_full = min(batched(pairs) for _ in range(3))
+    per_pair = (t_full - t_small) / (pairs - 16)
+    fixed = max(t_small - 16 * per_pair, 0.0)
+    out["batched_pairs_mode1"] = {"pairs": pairs, "call_ms": round(t_full * 1e3, 3),
+                                  "us_per_pair": round(t_full / pairs * 1e6, 3),
+                                  "call_16_pairs_ms": round(t_small * 1e3, 3)}
+    cpu = out["anchored_pairhmm"].get("cpu_reference_us_per_call")
+    if cpu and cpu * 1e-6 > per_pair:
+        # a batched call of N pairs (fixed + N per_pair) beats N scalar calls on one CPU core from N on
+        out["batched_break_even_pairs"] = int(math.ceil(fixed / (cpu * 1e-6 - per_pair)))
+    return out
+
+
 def genotype_bench(mb: float, depths, mode: int, steps: int, cpu_seconds: float, threads: int, tune: bool = True):
     """Forward + diploid genotype likelihood on the chromosome-scale sample
     (launch choices tuned on the sample itself unless tune is False)."""
@@ -479,6 +544,7 @@ def main():
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
+    ap.add_argument("--no-scalar", action="store_true", help="skip the scalar drop-in latency leg")
     ap.add_argument("--geno-mb", type=float, default=60.0,
                     help="chromosome-sample size for the genotype leg, Mb (0 = skip; default: the full 60 Mb sample)")
     ap.add_argument("--geno-depths", default="30,2", help="depth profiles of the genotype leg")
@@ -589,6 +655,11 @@ def main():
         fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)
         print("fwd+bwd: %s" % json.dumps(fwd_bwd), file=sys.stderr, flush=True)
 
+    scalar = None
+    if not args.no_scalar:
+        scalar = scalar_bench()
+        print("scalar drop-ins: %s" % json.dumps(scalar), file=sys.stderr, flush=True)
+
     genotype = None
     if args.geno_mb > 0:
         genotype = genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")], args.mode,
@@ -631,6 +702,7 @@ def main():
         "plain": plain,
         "fwd_bwd": fwd_bwd,
         "accuracy": accuracy,
+        "scalar_dropins": scalar,
         "genotype": genotype,
     }
     print(json.dumps(line), flush=True)

@@ -204,7 +204,8 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *                       while vd_prefix_sharing is on (the suffix part while
  *                       vd_suffix_sharing is on)
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
- * Defaults: 2, 0, 0, 0, 2, 4.  Values never change results beyond rounding
+ * Defaults: 2, 0, 0, 0, 2, 4, 1 (0 with VARDEPTH_CLASS_OVERLAP=0).  Values
+ * never change results beyond rounding
  * order (FP64-rescued pairs may differ in the last bit; the suffix
  * continuation agrees with full sweeps to FP32 rounding). */
 typedef struct vd_tuning {
@@ -214,6 +215,8 @@ typedef struct vd_tuning {
   int grad_blocks_per_cu[17];
   int fwd_share[17];
   int share_chunk;
+  int class_overlap;  /* 1: consecutive FP32 class kernels alternate between two streams
+                         (vd_class_overlap); 0: one stream */
 } vd_tuning;
 
 int vd_tuning_get(vd_tuning *t);
@@ -321,16 +324,20 @@ int vd_genotype_device(const vd_geno_batch *batch, const double *ll, const unsig
 
 /* Near-tie guard of partial_anchor.py:119-133 on a genotype batch (device
  * pointers): per region, gaps[2k] = largest - second largest of gl, gaps[2k+1]
- * the same of gl + log(prior) with the uniform haplotype prior of
- * whole_read.py:51 (1 for a == b, 2 otherwise, over n^2); flags[k] = 1 when
- * the smaller gap is <= margin_scale * max(1, molecules), molecules = rows
- * whose certainty (as written by vd_genotype_device) is not below
- * min_certainty.  partial_anchor.py uses margin_scale 1e-4 and recomputes a
- * flagged locus in FP64.  Regions with fewer than two pairs are not flagged.
- * nflagged (optional) synchronises and returns the count. */
+ * the same of gl + log(prior), and flags[k] = 1 when the smaller gap is
+ * <= margin_scale * max(1, molecules), molecules = rows whose certainty (as
+ * written by vd_genotype_device) is not below min_certainty.  The prior:
+ * with allele_freq (one per haplotype column of the batch, indexed like
+ * col_off) that of partial_anchor.py:56-57 -- each frequency floored at
+ * .001 and normalised over the region, prior(a, b) = f_a f_b (1 if a == b
+ * else 2) normalised; with allele_freq NULL the uniform haplotype prior of
+ * whole_read.py:51 (1 for a == b, 2 otherwise, over n^2).
+ * partial_anchor.py uses margin_scale 1e-4 and recomputes a flagged locus in
+ * FP64.  Regions with fewer than two pairs are not flagged.  nflagged
+ * (optional) synchronises and returns the count. */
 int vd_genotype_guard(const vd_geno_batch *batch, const double *gl, const double *certainty,
-                      double min_certainty, double margin_scale, signed char *flags, double *gaps,
-                      void *stream, int64_t *nflagged);
+                      const double *allele_freq, double min_certainty, double margin_scale, signed char *flags,
+                      double *gaps, void *stream, int64_t *nflagged);
 
 /* ---- baseline ----------------------------------------------------------- */
 

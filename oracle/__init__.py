@@ -73,6 +73,9 @@ def lib():
     L.vdo_genotype_regions.argtypes = [ctypes.c_int64, _i64p, ctypes.POINTER(ctypes.c_int32), _i64p, _dp,
                                        _u8p, ctypes.c_double, ctypes.c_double, _i64p, ctypes.c_int, _dp]
     L.vdo_genotype_regions.restype = ctypes.c_int64
+    L.vdo_guard.argtypes = [ctypes.c_int64, _i64p, ctypes.POINTER(ctypes.c_int32), _i64p, _i64p, _dp, _dp, _dp,
+                            ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int8), _dp]
+    L.vdo_guard.restype = ctypes.c_int64
     return L
 
 
@@ -222,6 +225,27 @@ def genotype_regions(row_off, ncol, mat_off, ll, mq, gl_off, min_certainty=float
                                      mq.ctypes.data_as(_u8p), float(min_certainty), float(prior),
                                      gl_off.ctypes.data_as(_i64p), int(threads), gl.ctypes.data_as(_dp))
     return gl[:total], int(bad)
+
+
+def guard(row_off, ncol, gl_off, gl, cert, af=None, min_certainty=float(np.log(3)), margin_scale=1e-4):
+    """partial_anchor.py:119-127 near-tie guard (prior from allele frequencies af,
+    one per haplotype column, partial_anchor.py:56-57; None: the uniform prior of
+    whole_read.py:51) -> (flags int8 [nregions], gaps float64 [nregions, 2])."""
+    row_off = np.ascontiguousarray(row_off, np.int64)
+    ncol = np.ascontiguousarray(ncol, np.int32)
+    gl_off = np.ascontiguousarray(gl_off, np.int64)
+    col_off = np.concatenate([[0], np.cumsum(ncol)]).astype(np.int64)
+    gl, cert = _f64(gl), _f64(cert)
+    nreg = len(ncol)
+    flags = np.zeros(max(nreg, 1), np.int8)
+    gaps = np.zeros(2 * max(nreg, 1), np.float64)
+    a = _f64(af) if af is not None else None
+    lib().vdo_guard(nreg, row_off.ctypes.data_as(_i64p), ncol.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                    col_off.ctypes.data_as(_i64p), gl_off.ctypes.data_as(_i64p), gl.ctypes.data_as(_dp),
+                    cert.ctypes.data_as(_dp), a.ctypes.data_as(_dp) if a is not None else None,
+                    float(min_certainty), float(margin_scale),
+                    flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), gaps.ctypes.data_as(_dp))
+    return flags[:nreg], gaps[:2 * nreg].reshape(-1, 2)
 
 
 # ---------------------------------------------------------------------------

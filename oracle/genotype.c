@@ -315,3 +315,71 @@ int64_t vdo_genotype_regions(int64_t nregions, const int64_t *row_off, const int
     free(th); free(cost); free(J.order);
     return J.invalid;
 }
+
+/* Near-tie guard of partial_anchor.py:119-127 over a genotype batch.
+ * Region k: n = ncol[k] haplotypes, pair outputs gl[gl_off[k] ..] in the
+ * order b = 0..n-1, a = 0..b (triangular.py:14), rows row_off[k] ..
+ * row_off[k+1] with certainty cert[row].  The haplotype prior is
+ *   af (per global haplotype column col_off[k] + c, or NULL): a_c = max(.001, af),
+ *   normalised; prior(a, b) = a_a a_b (1 if a == b else 2), normalised
+ *   (partial_anchor.py:56-57); with af NULL the uniform haplotype frequencies
+ *   1/n, i.e. prior 1/n^2 (hom) and 2/n^2 (het) (whole_read.py:51).
+ * gaps[2k] = largest - second largest gl, gaps[2k+1] the same of gl + log(prior);
+ * flags[k] = 1 when the smaller gap <= margin_scale * max(1, molecules), molecules
+ * = rows whose certainty is not below min_certainty.  Regions with fewer than
+ * two pairs get flag 0 and infinite gaps.  Sums run in index order. */
+int64_t vdo_guard(int64_t nregions, const int64_t *row_off, const int32_t *ncol, const int64_t *col_off,
+                  const int64_t *gl_off, const double *gl, const double *cert, const double *af,
+                  double min_certainty, double margin_scale, signed char *flags, double *gaps)
+{
+    int64_t nflag = 0;
+    double *w = NULL, *pr = NULL;
+    size_t cap = 0;
+    for (int64_t k = 0; k < nregions; ++k) {
+        const int64_t n = ncol[k], pairs = n * (n + 1) / 2;
+        flags[k] = 0;
+        gaps[2 * k] = gaps[2 * k + 1] = INFINITY;
+        if (pairs < 2) continue;
+        if ((size_t)pairs > cap) {
+            cap = (size_t)pairs;
+            w = (double *)realloc(w, sizeof(double) * cap);
+            pr = (double *)realloc(pr, sizeof(double) * cap);
+        }
+        if (af) {
+            double S = 0;
+            for (int64_t c = 0; c < n; ++c) S += fmax(.001, af[col_off[k] + c]);
+            int64_t q = 0;
+            double P = 0;
+            for (int64_t b = 0; b < n; ++b)
+                for (int64_t a = 0; a <= b; ++a, ++q) {
+                    const double fa = fmax(.001, af[col_off[k] + a]) / S, fb = fmax(.001, af[col_off[k] + b]) / S;
+                    pr[q] = fa * fb * (a == b ? 1.0 : 2.0);
+                    P += pr[q];
+                }
+            for (q = 0; q < pairs; ++q) pr[q] = log(pr[q] / P);
+        } else {
+            int64_t q = 0;
+            for (int64_t b = 0; b < n; ++b)
+                for (int64_t a = 0; a <= b; ++a, ++q) pr[q] = log((a == b ? 1.0 : 2.0) / (double)(n * n));
+        }
+        double a1 = -INFINITY, a2 = -INFINITY, b1 = -INFINITY, b2 = -INFINITY;
+        for (int64_t q = 0; q < pairs; ++q) {
+            const double v = gl[gl_off[k] + q], x = v + pr[q];
+            if (v > a1) { a2 = a1; a1 = v; } else if (v > a2) { a2 = v; }
+            if (x > b1) { b2 = b1; b1 = x; } else if (x > b2) { b2 = x; }
+        }
+        int64_t mol = 0;
+        for (int64_t r = row_off[k]; r < row_off[k + 1]; ++r) mol += !(cert[r] < min_certainty);
+        gaps[2 * k] = a1 - a2;
+        gaps[2 * k + 1] = b1 - b2;
+        const double margin = margin_scale * (mol > 1 ? (double)mol : 1.0);
+        const double lo = gaps[2 * k] < gaps[2 * k + 1] ? gaps[2 * k] : gaps[2 * k + 1];
+        if (lo <= margin) {
+            flags[k] = 1;
+            ++nflag;
+        }
+    }
+    free(w);
+    free(pr);
+    return nflag;
+}

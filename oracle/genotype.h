@@ -50,6 +50,13 @@ int64_t vdo_genotype_regions(int64_t nregions, const int64_t *row_off, const int
                              double min_certainty, double prior, const int64_t *gl_off,
                              int threads, double *gl);
 
+/* partial_anchor.py:119-127 near-tie guard with the allele-frequency prior of
+ * partial_anchor.py:56-57 (af per haplotype column) or, with af NULL, the
+ * uniform haplotype prior of whole_read.py:51.  Returns the flagged count. */
+int64_t vdo_guard(int64_t nregions, const int64_t *row_off, const int32_t *ncol, const int64_t *col_off,
+                  const int64_t *gl_off, const double *gl, const double *cert, const double *af,
+                  double min_certainty, double margin_scale, signed char *flags, double *gaps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -299,18 +299,11 @@ def test_near_tie_guard_recomputes_in_fp64(sample):
     g1 = guarded.cpu().numpy()
     nh = np.diff(b.region_haps)
     off = engine.gl_offsets(nh)
-    expected = []
-    for k in range(b.nregions):
-        gl = g0[off[k]:off[k + 1]]
-        if len(gl) < 2:
-            continue
-        n = int(nh[k])
-        prior = np.array([1.0 if a == c else 2.0 for c in range(n) for a in range(c + 1)])
-        lp = gl + np.log(prior / prior.sum())
-        molecules = int(np.sum(~(cert[b.region_reads[k]:b.region_reads[k + 1]] < math.log(3))))
-        gaps = [float(np.diff(np.partition(v, -2)[-2:])[0]) for v in (gl, lp)]
-        if min(gaps) <= 1e-4 * max(1, molecules):
-            expected.append(k)
+    # the oracle's restatement of the rule on the device's own likelihoods and certainties
+    flags, gaps = oracle.guard(b.region_reads, nh, off[:-1], g0, cert)
+    np.testing.assert_array_equal(report["flags"].cpu().numpy(), flags)
+    np.testing.assert_allclose(report["gaps"].cpu().numpy().reshape(-1, 2), gaps, rtol=1e-12, atol=1e-12)
+    expected = list(np.flatnonzero(flags))
     flagged = [r["region"] for r in report["regions"]]
     assert flagged == expected
     assert flagged and set(flagged) & set(range(0, b.nregions, 5))   # the planted ties are found
@@ -320,3 +313,30 @@ def test_near_tie_guard_recomputes_in_fp64(sample):
     for k in range(b.nregions):
         sl = slice(off[k], off[k + 1])
         np.testing.assert_array_equal(g1[sl], g64[sl] if k in set(flagged) else g0[sl])
+
+
+def test_near_tie_guard_allele_frequency_prior(sample):
+    """The guard with partial_anchor.py:56-57's allele-frequency prior: flags
+    and gaps equal the oracle's restatement on the same likelihoods, including
+    regions where only the prior makes the posterior gap close (a
+    heterozygote whose frequencies are set to cancel its likelihood lead)."""
+    import torch
+    b = sample.batch
+    dev = engine.DeviceBatch(b)
+    mq = torch.from_numpy(sample.mq).cuda()
+    dev.forward(mode=0)
+    rng = np.random.default_rng(17)
+    af = rng.uniform(0, 0.6, b.nhaps)
+    af[rng.random(b.nhaps) < 0.2] = 0.0          # unknown frequencies: floored at .001
+    guarded, report = dev.genotype_guarded(mq, mode=0, allele_freq=af)
+    gl, _, cert = dev.genotype(mq, values=True)
+    nh = np.diff(b.region_haps)
+    off = engine.gl_offsets(nh)
+    flags, gaps = oracle.guard(b.region_reads, nh, off[:-1], gl.cpu().numpy(), cert.cpu().numpy(), af=af)
+    np.testing.assert_array_equal(report["flags"].cpu().numpy(), flags)
+    np.testing.assert_allclose(report["gaps"].cpu().numpy().reshape(-1, 2), gaps, rtol=1e-12, atol=1e-12)
+    assert report["prior"] == "allele_frequency"
+    uniform = oracle.guard(b.region_reads, nh, off[:-1], gl.cpu().numpy(), cert.cpu().numpy())[1]
+    assert np.any(np.abs(gaps[:, 1] - uniform[:, 1]) > 1e-3)     # the prior changes the posterior gaps
+    with pytest.raises(ValueError, match="one frequency per haplotype"):
+        dev.genotype_guarded(mq, allele_freq=af[:-1])

@@ -306,3 +306,35 @@ def test_haplotype_length_limit_checked_before_the_device():
     g = workloads.from_regions([(["ACGT"], [b"\x1e" * 4], ["A" * (glim + 1)])])
     with pytest.raises(_lib.VdhmmError, match="gradient kernels"):
         engine.grad(g)
+
+
+def test_tuning_save_load_resets_absent_fields(tmp_path):
+    """autotune.load applies every saved field and gives the ones an entry
+    lacks their defaults, the stream layout included (vd_tuning.class_overlap)."""
+    import json
+    from vardepth_amd import _lib, autotune
+    base = autotune.get()
+    try:
+        t = _lib.VdTuning.from_buffer_copy(base)
+        t.class_overlap = 0
+        t.share_chunk = 16
+        t.fwd_share[7] = 1
+        autotune.set_tuning(t)
+        assert _lib.lib().vd_class_overlap(-1) == 0 and autotune.get().class_overlap == 0
+        path = tmp_path / "tune.json"
+        autotune.save(path, "a")
+        saved = json.loads(path.read_text())["a"]
+        assert saved["class_overlap"] == 0 and saved["share_chunk"] == 16
+        # an entry without the stream layout and the chunk: both return to their defaults
+        del saved["class_overlap"], saved["share_chunk"]
+        path.write_text(json.dumps({"b": saved}))
+        assert autotune.load(path, "b")
+        got = autotune.get()
+        assert got.class_overlap == 1 and got.share_chunk == 4 and got.fwd_share[7] == 1
+        assert not autotune.load(path, "missing")
+        bad = _lib.VdTuning.from_buffer_copy(base)
+        bad.class_overlap = 2
+        with pytest.raises(_lib.VdhmmError, match="class_overlap"):
+            autotune.set_tuning(bad)
+    finally:
+        autotune.set_tuning(base)

@@ -183,3 +183,100 @@ def test_planted_mistake_is_caught():
     a, b = m[:, 0], m[:, 1]
     bad = math.log(sum(w * math.exp(np.sum(np.log(x * a + (1 - x) * b))) for x, w in zip(bal, bw_bad)))
     assert abs(bad - good[1]) > 1e-3
+
+
+# ---- near-tie guard (partial_anchor.py:56-57,119-127; whole_read.py:51) -------------------
+
+def _one_region(gl, rows_cert, n):
+    return dict(row_off=[0, len(rows_cert)], ncol=[n], gl_off=[0], gl=np.asarray(gl, float),
+                cert=np.asarray(rows_cert, float))
+
+
+def test_guard_hand_cases_uniform_prior():
+    # n = 2: pairs (0,0), (0,1), (1,1); uniform prior 1/4, 1/2, 1/4
+    gl = [-10.0, -10.00005, -30.0]
+    f, g = oracle.guard(**_one_region(gl, [5.0], 2))
+    assert g[0, 0] == pytest.approx(5e-5, rel=1e-9)
+    assert g[0, 1] == pytest.approx(math.log(2) - 5e-5, rel=1e-12)   # the het pair gains log 2
+    assert f[0] == 1                                                  # 5e-5 <= 1e-4 * max(1, 1)
+    f, _ = oracle.guard(**_one_region([-10.0, -10.0002, -30.0], [5.0], 2))
+    assert f[0] == 0                                                  # 2e-4 > 1e-4
+    # molecules = rows whose certainty is not below ln 3: the margin scales with them
+    f, _ = oracle.guard(**_one_region([-10.0, -10.00025, -30.0], [5.0, 5.0, 0.5], 2))
+    assert f[0] == 0                                                  # two molecules: 2.5e-4 > 2e-4
+    f, _ = oracle.guard(**_one_region([-10.0, -10.00025, -30.0], [5.0, 5.0, 5.0], 2))
+    assert f[0] == 1                                                  # three molecules: 2.5e-4 <= 3e-4
+    f, _ = oracle.guard(**_one_region([-10.0, -10.00025, -30.0], [5.0, 5.0, float("nan")], 2))
+    assert f[0] == 1                                                  # NaN certainty counts (not below ln 3)
+    # the posterior gap alone can flag: gl clearly separated, the prior closes the gap
+    gl = [-10.0, -10.0 - math.log(2), -40.0]
+    f, g = oracle.guard(**_one_region(gl, [5.0], 2))
+    assert g[0, 0] == pytest.approx(math.log(2)) and g[0, 1] == pytest.approx(0.0, abs=1e-14) and f[0] == 1
+    # a single pair is never flagged
+    f, g = oracle.guard(**_one_region([-3.0], [5.0], 1))
+    assert f[0] == 0 and np.isinf(g).all()
+
+
+def test_guard_allele_frequency_prior():
+    n = 3
+    pairs = [(a, b) for b in range(n) for a in range(b + 1)]
+    # equal frequencies (including floored zeros) reproduce the uniform prior exactly
+    rng = np.random.default_rng(3)
+    gl = rng.normal(-20, 3, len(pairs))
+    base = oracle.guard(**_one_region(gl, [5.0], n))
+    for af in ([0.2, 0.2, 0.2], [0.0, 0.0, 0.0], [0.0005, 0.001, 0.0]):
+        f, g = oracle.guard(**_one_region(gl, [5.0], n), af=af)
+        np.testing.assert_allclose(g, base[1], rtol=0, atol=1e-12)
+    # a constant gl: the posterior gap is the log ratio of the two largest prior weights
+    af = np.array([0.7, 0.2, 0.1])
+    p = np.array([af[a] * af[b] * (1 if a == b else 2) for a, b in pairs])
+    top = np.sort(p)[::-1]
+    f, g = oracle.guard(**_one_region(np.full(len(pairs), -5.0), [5.0], n), af=af)
+    assert g[0, 0] == 0.0 and f[0] == 1
+    assert g[0, 1] == pytest.approx(math.log(top[0] / top[1]), rel=1e-12)   # 0.49 vs 0.28
+    # frequencies are floored at .001 before normalising (partial_anchor.py:56)
+    f1, g1 = oracle.guard(**_one_region(gl, [5.0], n), af=[0.5, 0.5, 0.0])
+    f2, g2 = oracle.guard(**_one_region(gl, [5.0], n), af=[0.5, 0.5, 0.001])
+    np.testing.assert_array_equal(g1, g2)
+    # scaling all frequencies leaves the prior unchanged
+    f3, g3 = oracle.guard(**_one_region(gl, [5.0], n), af=[5.0, 5.0, 0.01])
+    np.testing.assert_allclose(g3, g2, rtol=0, atol=1e-12)
+
+
+def _numpy_form(lh, artifact_prior):
+    """triangular.py:13-24 evaluated with numpy (the reference's canonical form)."""
+    balance = (.2, .3, .4, .5, .6, .7, .8)
+    bw = (.025, .1, .225, .3, .225, .1, .025)
+    theta = (.02, .1, .2, .3)
+    tw = (.4, .35, .2, .05)
+    n = lh.shape[1]
+    a, b = np.array([(a, b) for b in range(n) for a in range(b + 1)], dtype=np.int64).T
+    la, lb = lh[:, a], lh[:, b]
+    if artifact_prior is None:
+        return np.log(np.maximum(1e-300, (la + lb) / 2)).sum(axis=0)
+    clean = np.array([np.log(np.maximum(1e-300, w * la + (1 - w) * lb)).sum(axis=0) + math.log(p)
+                      for w, p in zip(balance, bw)])
+    clean = np.logaddexp.reduce(clean, axis=0)
+    if artifact_prior:
+        art = []
+        for t, w in zip(theta, tw):
+            for c in range(n):
+                art.append(np.log(np.maximum(1e-300, (1 - t) * (la + lb) / 2 + t * lh[:, c, None])).sum(axis=0)
+                           + math.log(w / n))
+        art = np.logaddexp.reduce(np.array(art), axis=0)
+        clean = np.logaddexp(math.log1p(-artifact_prior) + clean, math.log(artifact_prior) + art)
+    return clean
+
+
+def test_numpy_canonical_form_equals_mixture_general():
+    """The reference's two forms of the diploid likelihood -- the numpy one of
+    triangular.py:13-24 and mixture_general.cpp -- agree to the last bits of
+    their logs, so the device mixture (pinned to mixture_general) serves both."""
+    rng = np.random.default_rng(61)
+    for n in (1, 2, 3, 7, 16):
+        for rows in (0, 1, 9, 40):
+            lh = rng.choice([1.0, 0.3, 1e-3, 1e-200, 0.0], (rows, n)) * rng.uniform(0.5, 1, (rows, n))
+            for prior in (None, 0.0, 1e-3, 0.3):
+                want = _numpy_form(lh, prior)
+                got = oracle.mixture_general(lh, prior, triangular=True)
+                np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-12)

@@ -61,10 +61,11 @@ class VdStats(ctypes.Structure):
 class VdTuning(ctypes.Structure):
     _fields_ = [("fwd_unroll", ctypes.c_int * 17), ("fwd_blocks_per_cu", ctypes.c_int * 17),
                 ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17),
-                ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int)]
+                ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int), ("class_overlap", ctypes.c_int)]
+    SCALARS = ("share_chunk", "class_overlap")
 
     def as_dict(self):
-        return {name: (list(getattr(self, name)) if name != "share_chunk" else int(self.share_chunk))
+        return {name: (int(getattr(self, name)) if name in self.SCALARS else list(getattr(self, name)))
                 for name, _ in self._fields_}
 
 
@@ -169,9 +170,9 @@ def lib() -> ctypes.CDLL:
                                      ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(VdGenoStats)]
     L.vd_genotype_device.restype = ctypes.c_int
-    L.vd_genotype_guard.argtypes = [ctypes.POINTER(VdGenoBatch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
-                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                    ctypes.POINTER(ctypes.c_int64)]
+    L.vd_genotype_guard.argtypes = [ctypes.POINTER(VdGenoBatch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
     L.vd_genotype_guard.restype = ctypes.c_int
     L.vd_forward_device_noalloc.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,

@@ -11,7 +11,7 @@ passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
   per CU); before the per-class sweep, the haplotypes per read unit under
   sharing (4, 8 or 16), timed for the whole forward with the class kernels
   on two streams and on one; after it, the stream layout
-  (``vd_class_overlap``) with the final choices. Two streams fill each
+  (``vd_tuning.class_overlap``) with the final choices. Two streams fill each
   kernel's tail, but where one class holds most of the work they share the
   CUs with it for its whole run, and their time varies more: they are kept
   only when more than 2% faster;
@@ -23,7 +23,10 @@ continuation (which sums the suffix columns through an adjoint and agrees
 with full sweeps to FP32 rounding), so results are otherwise identical for
 every choice; only the time differs.  ``tune_forward`` / ``tune_grad`` apply
 the per-class best and return a report; ``save`` / ``load`` keep a choice
-in a JSON file keyed by device and mode.
+(every field of ``vd_tuning``, the stream layout included) in a JSON file
+keyed by workload and mode, and ``load`` gives a field the entry lacks its
+default.  The choices are the library's, for every later call of the
+process: tune on the batch that will run, or load its saved choice.
 """
 from __future__ import annotations
 
@@ -112,10 +115,10 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
     pair_ms = {}
     base = get()
     for overlap in (1, 0):  # whole-forward time of each stream layout x read-unit size (sharing on everywhere)
-        lib.vd_class_overlap(overlap)
         for chunk in CHUNK_CANDIDATES:
             t = _lib.VdTuning.from_buffer_copy(base)
             t.share_chunk = chunk
+            t.class_overlap = overlap
             for r in range(1, 17):
                 t.fwd_share[r] = 2
             set_tuning(t)
@@ -146,7 +149,7 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         overlap_ms[o] = min(dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"]
                             for _ in range(max(reps, 2)))
     overlap = 1 if overlap_ms[1] < 0.98 * overlap_ms[0] else 0
-    lib.vd_class_overlap(overlap)
+    lib.vd_class_overlap(overlap)   # kept in vd_tuning.class_overlap with the other choices
     return {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
             "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
             "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
@@ -172,32 +175,42 @@ def tune_grad(dev_batch, mode=0, reps=1, candidates=GRAD_CANDIDATES):
             "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
 
 
+def defaults() -> _lib.VdTuning:
+    """The library's launch choices before any tuning (include/vdhmm.h)."""
+    t = _lib.VdTuning()
+    for r in range(17):
+        t.fwd_unroll[r], t.fwd_blocks_per_cu[r], t.fwd_share[r] = FWD_CANDIDATES[0]
+        t.grad_variant[r], t.grad_blocks_per_cu[r] = GRAD_CANDIDATES[0]
+    t.share_chunk = CHUNK_CANDIDATES[0]
+    t.class_overlap = 1
+    return t
+
+
 def save(path, key: str):
     p = Path(path)
     data = json.loads(p.read_text()) if p.exists() else {}
     data[key] = get().as_dict()
-    data[key]["class_overlap"] = int(_lib.lib().vd_class_overlap(-1))
     p.write_text(json.dumps(data, indent=1, sort_keys=True))
 
 
 def load(path, key: str) -> bool:
+    """Apply the choice kept under ``key``; a field the entry lacks takes its
+    default, so no choice of an earlier tuning leaks into the loaded one."""
     p = Path(path)
     if not p.exists():
         return False
     data = json.loads(p.read_text())
     if key not in data:
         return False
-    t = get()
+    t = defaults()
     for name, _ in t._fields_:
         if name not in data[key]:
             continue
-        if name == "share_chunk":
-            t.share_chunk = int(data[key][name])
-            continue
-        arr = getattr(t, name)
-        for i, v in enumerate(data[key][name]):
-            arr[i] = int(v)
+        if name in _lib.VdTuning.SCALARS:
+            setattr(t, name, int(data[key][name]))
+        else:
+            arr = getattr(t, name)
+            for r, v in enumerate(data[key][name][:17]):
+                arr[r] = int(v)
     set_tuning(t)
-    if "class_overlap" in data[key]:
-        _lib.lib().vd_class_overlap(int(data[key]["class_overlap"]))
     return True

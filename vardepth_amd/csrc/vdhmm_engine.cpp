@@ -1014,8 +1014,9 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   return geno_stats(st, s, true);
 }
 
-int guard_device_locked(const vd_geno_batch* b, const double* gl, const double* cert, double min_cert,
-                        double margin_scale, int8_t* flags, double* gaps, hipStream_t st, int64_t* nflagged) {
+int guard_device_locked(const vd_geno_batch* b, const double* gl, const double* cert, const double* af,
+                        double min_cert, double margin_scale, int8_t* flags, double* gaps, hipStream_t st,
+                        int64_t* nflagged) {
   if (int rc = check_geno_batch(b)) return rc;
   if (std::isnan(min_cert) || !(margin_scale >= 0)) return fail("vdhmm: invalid guard margin or certainty bound");
   if (b->nregions > 0 && (!gl || !cert || !flags || !gaps)) return fail("vdhmm: null guard argument");
@@ -1023,7 +1024,7 @@ int guard_device_locked(const vd_geno_batch* b, const double* gl, const double* 
   StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   if (int rc = geno_zero_ctr(st)) return rc;
-  HIPCHK(vd::launch_guard(geno_dev(b), gl, cert, min_cert, margin_scale, flags, gaps,
+  HIPCHK(vd::launch_guard(geno_dev(b), gl, cert, af, min_cert, margin_scale, flags, gaps,
                           gg.ctr.as<unsigned long long>(), st));
   if (nflagged) {
     HIPCHK(hipStreamSynchronize(st));
@@ -1340,11 +1341,12 @@ int vd_genotype_device(const vd_geno_batch* batch, const double* ll, const unsig
   return rc;
 }
 
-int vd_genotype_guard(const vd_geno_batch* batch, const double* gl, const double* certainty, double min_certainty,
-                      double margin_scale, signed char* flags, double* gaps, void* stream, int64_t* nflagged) {
+int vd_genotype_guard(const vd_geno_batch* batch, const double* gl, const double* certainty,
+                      const double* allele_freq, double min_certainty, double margin_scale, signed char* flags,
+                      double* gaps, void* stream, int64_t* nflagged) {
   std::lock_guard<std::mutex> lk(g_mu);
   CaptureScope capture((hipStream_t)stream);
-  return guard_device_locked(batch, gl, certainty, min_certainty, margin_scale, (int8_t*)flags, gaps,
+  return guard_device_locked(batch, gl, certainty, allele_freq, min_certainty, margin_scale, (int8_t*)flags, gaps,
                              (hipStream_t)stream, nflagged);
 }
 
@@ -1358,6 +1360,7 @@ int vd_tuning_get(vd_tuning* t) {
   std::lock_guard<std::mutex> lk(g_mu);
   tune_defaults();
   *t = g.tune;
+  t->class_overlap = g.overlap ? 1 : 0;
   return 0;
 }
 
@@ -1370,9 +1373,11 @@ int vd_tuning_set(const vd_tuning* t) {
     if (t->fwd_share[r] < 0 || t->fwd_share[r] > 3) return fail("vdhmm: fwd_share must be 0, 1, 2 or 3");
   }
   if (t->share_chunk < 1 || t->share_chunk > 64) return fail("vdhmm: share_chunk must be in 1..64");
+  if (t->class_overlap != 0 && t->class_overlap != 1) return fail("vdhmm: class_overlap must be 0 or 1");
   std::lock_guard<std::mutex> lk(g_mu);
   g.tune = *t;
   g.tune_init = true;
+  g.overlap = t->class_overlap != 0;
   return 0;
 }
 

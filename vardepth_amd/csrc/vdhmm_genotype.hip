@@ -337,9 +337,21 @@ __global__ __launch_bounds__(256) void k_mixture(GenoDev g, const double* __rest
 // (a != b) over n^2 (whole_read.py:51), against margin_scale * max(1, molecules),
 // molecules = rows whose certainty is not below min_cert.  Regions with fewer
 // than two pairs are never flagged.
-__global__ void k_guard(GenoDev g, const double* __restrict__ gl, const double* __restrict__ cert, double min_cert,
-                        double margin_scale, int8_t* __restrict__ flag, double* __restrict__ gaps,
-                        unsigned long long* __restrict__ ctr) {
+// Haplotype prior of the guard: with allele frequencies af (per haplotype
+// column) the one of partial_anchor.py:56-57 -- a_c = max(.001, af) normalised,
+// prior(a, b) = a_a a_b (1 if a == b else 2) normalised -- else the uniform
+// frequencies of whole_read.py:51 (1/n^2 hom, 2/n^2 het).  Sums run in index
+// order.
+__device__ __forceinline__ double guard_log_prior(const double* __restrict__ af, int64_t c0, int64_t n, int64_t a,
+                                                  int64_t b, double S, double P) {
+  if (!af) return log((a == b ? 1.0 : 2.0) / (double)(n * n));
+  const double fa = fmax(.001, af[c0 + a]) / S, fb = fmax(.001, af[c0 + b]) / S;
+  return log(fa * fb * (a == b ? 1.0 : 2.0) / P);
+}
+
+__global__ void k_guard(GenoDev g, const double* __restrict__ gl, const double* __restrict__ cert,
+                        const double* __restrict__ af, double min_cert, double margin_scale,
+                        int8_t* __restrict__ flag, double* __restrict__ gaps, unsigned long long* __restrict__ ctr) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < g.nregions;
        k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t n = g.col_off[k + 1] - g.col_off[k];
@@ -351,13 +363,22 @@ __global__ void k_guard(GenoDev g, const double* __restrict__ gl, const double* 
     if (pairs < 2 || go < 0 || go + pairs > g.gl_size || r0 < 0 || r1 > g.nrows) continue;
     int64_t mol = 0;
     for (int64_t r = r0; r < r1; ++r) mol += !(cert[r] < min_cert);
-    const double lw_hom = log(1.0 / (double)(n * n)), lw_het = log(2.0 / (double)(n * n));
+    const int64_t c0 = g.col_off[k];
+    double S = 0.0, P = 0.0;
+    if (af) {
+      for (int64_t c = 0; c < n; ++c) S += fmax(.001, af[c0 + c]);
+      for (int64_t bb = 0; bb < n; ++bb)
+        for (int64_t aa = 0; aa <= bb; ++aa) {
+          const double fa = fmax(.001, af[c0 + aa]) / S, fb = fmax(.001, af[c0 + bb]) / S;
+          P += fa * fb * (aa == bb ? 1.0 : 2.0);
+        }
+    }
     double a1 = -__builtin_inf(), a2 = -__builtin_inf(), b1 = -__builtin_inf(), b2 = -__builtin_inf();
     int64_t q = 0;
     for (int64_t bb = 0; bb < n; ++bb)
       for (int64_t aa = 0; aa <= bb; ++aa, ++q) {
         const double v = gl[go + q];
-        const double w = v + (aa == bb ? lw_hom : lw_het);
+        const double w = v + guard_log_prior(af, c0, n, aa, bb, S, P);
         if (v > a1) { a2 = a1; a1 = v; } else if (v > a2) { a2 = v; }
         if (w > b1) { b2 = b1; b1 = w; } else if (w > b2) { b2 = w; }
       }
@@ -383,11 +404,12 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, double min_cert, double margin_scale,
-                        int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st) {
+hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, const double* af, double min_cert,
+                        double margin_scale, int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st) {
   if (g.nregions <= 0) return hipSuccess;
   const int blocks = (int)((g.nregions + 255) / 256 < 4096 ? (g.nregions + 255) / 256 : 4096);
-  hipLaunchKernelGGL(k_guard, dim3(blocks), dim3(256), 0, st, g, gl, cert, min_cert, margin_scale, flag, gaps, ctr);
+  hipLaunchKernelGGL(k_guard, dim3(blocks), dim3(256), 0, st, g, gl, cert, af, min_cert, margin_scale, flag, gaps,
+                     ctr);
   return hipGetLastError();
 }
 

@@ -203,8 +203,8 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 hipError_t noalloc_scratch(int dev, void** ptr, size_t* bytes);
 hipError_t launch_put_tables(const QTab<float>& t32, const QTab<double>& t64, const int32_t* E, QTab<float>* d32,
                              QTab<double>* d64, int32_t* dE, hipStream_t st);
-hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, double min_cert, double margin_scale,
-                        int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st);
+hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, const double* af, double min_cert,
+                        double margin_scale, int8_t* flag, double* gaps, unsigned long long* ctr, hipStream_t st);
 hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq, const double* mq_err,
                            double min_cert, double* values, double* cert, double* mol, int8_t* keep,
                            unsigned long long* ctr, int cus, hipStream_t st);

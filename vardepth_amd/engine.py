@@ -229,31 +229,42 @@ class DeviceBatch:
         return res[0] if len(res) == 1 else tuple(res)
 
     def genotype_guarded(self, mq, mode=0, min_certainty=GENO_MIN_CERTAINTY, prior=GENO_PRIOR,
-                         margin_scale=GUARD_MARGIN, stream=None):
+                         margin_scale=GUARD_MARGIN, stream=None, allele_freq=None):
         """The genotype chain with the near-tie guard of partial_anchor.py:119-133.
 
         After :meth:`genotype` on the mixed-precision likelihoods in self.out
         (run ``forward(mode=mode)`` first), every region whose two best
-        genotypes -- by likelihood, or by likelihood plus the uniform
-        haplotype prior -- are within ``margin_scale * max(1, molecules)`` is
-        recomputed from an all-FP64 forward of its reads and haplotypes, and
-        its outputs replaced.  Returns (gl device tensor, report): the report
-        lists the flagged regions with their observed gaps and margins."""
+        genotypes -- by likelihood, or by likelihood plus the haplotype prior
+        -- are within ``margin_scale * max(1, molecules)`` is recomputed from
+        an all-FP64 forward of its reads and haplotypes, and its outputs
+        replaced.  The prior comes from ``allele_freq`` (float64, one per
+        haplotype of the batch: partial_anchor.py:56-57's panel frequencies,
+        floored at .001) or, when None, is whole_read.py:51's uniform one.
+        Returns (gl device tensor, report): the report lists the flagged
+        regions with their observed gaps and margins."""
         torch = self.torch
         s = stream if stream is not None else torch.cuda.current_stream()
         gl, _, cert = self.genotype(mq, min_certainty, prior, stream=s, values=True)
         nreg = self.batch.nregions
+        af = None
+        if allele_freq is not None:
+            af = torch.as_tensor(allele_freq, dtype=torch.float64).to(self.out.device).contiguous()
+            if af.numel() != self.batch.nhaps:
+                raise ValueError("allele_freq must hold one frequency per haplotype of the batch")
         flags = torch.empty(max(nreg, 1), dtype=torch.int8, device=self.out.device)
         gaps = torch.empty(2 * max(nreg, 1), dtype=torch.float64, device=self.out.device)
         nflag = ctypes.c_int64()
         code = _lib.lib().vd_genotype_guard(ctypes.byref(self.gb), ctypes.c_void_p(gl.data_ptr()),
-                                            ctypes.c_void_p(cert.data_ptr()), float(min_certainty),
+                                            ctypes.c_void_p(cert.data_ptr()),
+                                            ctypes.c_void_p(af.data_ptr()) if af is not None else None,
+                                            float(min_certainty),
                                             float(margin_scale), ctypes.c_void_p(flags.data_ptr()),
                                             ctypes.c_void_p(gaps.data_ptr()), ctypes.c_void_p(s.cuda_stream),
                                             ctypes.byref(nflag))
         _lib.check(code, "vd_genotype_guard")
         report = {"backend": "hip_float64", "reason": "near_tied_likelihood_or_posterior",
-                  "margin_scale": margin_scale, "regions": []}
+                  "margin_scale": margin_scale, "prior": "allele_frequency" if af is not None else "uniform",
+                  "flags": flags[:nreg], "gaps": gaps[:2 * nreg], "regions": []}
         if nflag.value == 0:
             return gl, report
         ids = np.flatnonzero(flags[:nreg].cpu().numpy())
