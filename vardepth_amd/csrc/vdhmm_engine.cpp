@@ -141,6 +141,7 @@ struct Engine {
   DevBuf in_read, in_qual, in_roff, in_hap, in_hoff, in_rreg, in_hreg, in_ooff, in_out;
   // plan scratch
   DevBuf read_cls, read_exp, class_off, read_perm, unit_cnt, unit_bucket, f64_list, ctr, ws64;
+  DevBuf f64_sorted, f64_bins;  // the FP64 list binned by shape, and its bin counters
   // parameter tables
   DevBuf qE, tab32, tab64, gtab;
   // gradient scratch: checkpoints, per-block partial gradients, staged weights / gradient
@@ -391,6 +392,8 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   }
   const int blocks64 = std::max(1, g.cus * std::max(1, std::min(g.occ64[anchored], 16)));
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
+  HIPCHK(g.f64_sorted.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
+  HIPCHK(g.f64_bins.ensure(2 * vd::F64_BINS_TOTAL * sizeof(unsigned long long)));
   int trie = 0;  // class mask of the read units with prefix sharing
   if (g.share && !force64)
     for (int r = 1; r <= vd::MAX_R32; ++r) trie |= g.tune.fwd_share[r] ? (1 << r) : 0;
@@ -449,9 +452,9 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks32s, unroll32,
-                             blocks64,
-                             g.ws64.as<double>(), dout, st, stats ? g.ev : nullptr, timing ? g.ev_cls : nullptr,
-                             two ? &lane2 : nullptr));
+                             blocks64, g.ws64.as<double>(), g.f64_sorted.as<vd::PairRef>(),
+                             g.f64_bins.as<unsigned long long>(), dout, st, stats ? g.ev : nullptr,
+                             timing ? g.ev_cls : nullptr, two ? &lane2 : nullptr));
   if (timing && !force64) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     for (int r = 1; r <= vd::MAX_R32; ++r) {
@@ -572,7 +575,8 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   p.trie = 0;
   p.trie_chunk = g.tune.share_chunk;
   HIPCHK(vd::launch_pipeline(batch_dev(db), p, dE, d32, d64, anchored, force64, smax, blocks32, blocks32, unroll32,
-                             blocks64, reinterpret_cast<double*>(B + o_ws), dout, st, nullptr, nullptr, nullptr));
+                             blocks64, reinterpret_cast<double*>(B + o_ws), nullptr, nullptr, dout, st, nullptr,
+                             nullptr, nullptr));
   return 0;
 }
 

@@ -184,8 +184,12 @@ struct SideLane {
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            const int* blocks32s,
-                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st,
-                           hipEvent_t* ev, hipEvent_t* ev_cls, const SideLane* lane2);
+                           const int* unroll32, int blocks64, double* ws64, PairRef* f64_sorted,
+                           unsigned long long* f64_bins, double* out, hipStream_t st, hipEvent_t* ev,
+                           hipEvent_t* ev_cls, const SideLane* lane2);
+// FP64 list binning (f64_sorted / f64_bins non-null): capacity of the sorted
+// copy = the list's, bins = 2 * F64_BINS_TOTAL counters
+constexpr int F64_BINS_TOTAL = 16 * 32;
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
 size_t lds_grad32(int smax);

@@ -30,6 +30,8 @@
 // terminal falls below 2^-100 is appended to the FP64 work list.  The FP64
 // kernel uses the same recurrence with 64-row tiles, renormalising the
 // boundary row between tiles, so it covers any read length.
+#include <type_traits>
+
 #include "vdhmm_device.h"
 
 namespace vd {
@@ -980,6 +982,102 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, 
 // 2^CEXP64 before the next tile, so any read length stays in range: a tile
 // decays by at most 64 ln(4/x) nats along the insertion path.
 
+// One tile's sweep of the FP64 forward.  FIRST: the boundary row is the
+// model's start (free starts, or the anchored origin); otherwise it comes
+// from the previous tile's last row, pre-scaled by that tile's
+// renormalisation as it is loaded (16 columns per vector load, one chunk
+// ahead, handed down the lanes by DPP row_shl:1).  LAST: the tile holds the
+// read's last row and accumulates the terminal; otherwise lane 15 stores the
+// tile's last row and its maximum.  Steps before the shortest haplotype's end
+// run without the end-of-haplotype test; both loops advance two steps per
+// iteration so the loop-carried state stays in place.
+template <bool ANCH, bool FIRST, bool LAST>
+__device__ __forceinline__ void tile64(const GroupPair& gp, const uint16_t* cl, int tmain, int tend,
+                                       const double (&gd)[R64], const double (&bd)[R64], const double (&mu)[R64],
+                                       const double (&be)[R64], const uint32_t (&rc)[R64], double gam,
+                                       double gam_last, double xx, double sc1, double sc2,
+                                       const double* __restrict__ inP,
+                                       const double* __restrict__ inQ, double* __restrict__ oP,
+                                       double* __restrict__ oQ, double& acc, double& rmax, double& latch,
+                                       double& bmax) {
+  constexpr int R = R64;
+  const int l = threadIdx.x & 15;
+  double M[R], D[R], Q[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.0;
+  double Plast = 0.0, Xprev = 0.0;
+  const double C = 0x1p500;
+  double bP = 0.0, bQ = 0.0, nP = 0.0, nQ = 0.0;
+  if (!FIRST) {
+    if (l - 1 <= gp.m_max) {
+      bP = inP[l + 16] * sc1 * sc2;
+      bQ = inQ[l + 16] * sc1 * sc2;
+    }
+    if (l + 15 <= gp.m_max) {
+      nP = inP[l + 32] * sc1 * sc2;
+      nQ = inQ[l + 32] * sc1 * sc2;
+    }
+  }
+  auto step = [&](int t, auto tail) {
+    constexpr bool TAIL = decltype(tail)::value;
+    const uint32_t code = cl[t];
+    double pinj, qinj;
+    if (FIRST) {
+      if (ANCH) {
+        pinj = qinj = t == 0 ? C : 0.0;
+      } else {
+        pinj = (t == 0 || t - 1 > gp.m) ? 0.0 : C;
+        qinj = t == 0 ? 0.0 : C;
+      }
+    } else {
+      if (t > 0 && (t & 15) == 0) {
+        bP = nP;
+        bQ = nQ;
+        nP = nQ = 0.0;
+        if (t + l + 15 <= gp.m_max) {
+          nP = inP[t + l + 32] * sc1 * sc2;
+          nQ = inQ[t + l + 32] * sc1 * sc2;
+        }
+      }
+      pinj = bP;
+      qinj = bQ;
+      bP = dpp_shl1(0.0, bP);
+      bQ = dpp_shl1(0.0, bQ);
+    }
+    const double Pin = dpp_shr1(pinj, Plast);
+    const double Xn = dpp_shr1(qinj, Q[R - 1]);
+    Plast = adiag_step<double, R, TAIL>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
+    Xprev = Xn;
+    if (LAST) {
+      if (ANCH) {
+        rmax = fmax(rmax, Q[R - 1]);
+        latch = (t - l - 1 == gp.m) ? Q[R - 1] : latch;
+      } else {
+        acc += Plast;
+      }
+    } else {
+      if (l == GROUP - 1) {
+        oP[t + 1] = Plast;  // column t-16 -> slot t+1
+        oQ[t + 1] = Q[R - 1];
+      }
+      bmax = fmax(bmax, fmax(Plast, Q[R - 1]));
+    }
+  };
+  using head_t = std::integral_constant<bool, false>;
+  using tail_t = std::integral_constant<bool, true>;
+  int t = 0;
+  for (; t + 1 < tmain; t += 2) {
+    step(t, head_t{});
+    step(t + 1, head_t{});
+  }
+  if (t < tmain) step(t++, head_t{});
+  for (; t + 1 <= tend; t += 2) {
+    step(t, tail_t{});
+    step(t + 1, tail_t{});
+  }
+  if (t <= tend) step(t, tail_t{});
+}
+
 template <bool ANCH>
 __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, const QTab<double>* __restrict__ tab,
                                        uint16_t* codes, int smax, const GroupPair gp, double* __restrict__ ws,
@@ -1039,85 +1137,27 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
       }
       gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
     }
-    double M[R], D[R], Q[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.0;
-    double Plast = 0.0, Xprev = 0.0, bmax = 0.0;
     const double* inP = bufP[(tile + 1) & 1];
     const double* inQ = bufQ[(tile + 1) & 1];
     double* oP = bufP[tile & 1];
     double* oQ = bufQ[tile & 1];
-    const double C = 0x1p500;
-    // The previous tile's last row enters lane 0 at step t as column t-1
-    // (slot t+16; zero past m_max).  It is read 16 steps per vector load:
-    // from step t0 (a multiple of 16) lane l holds step t0+l's values, and
-    // they shift down one lane per step; the next 16 are loaded one chunk
-    // ahead, so no step waits on a load.
-    double bP = 0.0, bQ = 0.0, nP = 0.0, nQ = 0.0;
-    if (!first) {
-      if (l - 1 <= gp.m_max) {
-        bP = inP[l + 16];
-        bQ = inQ[l + 16];
-      }
-      if (l + 15 <= gp.m_max) {
-        nP = inP[l + 32];
-        nQ = inQ[l + 32];
-      }
-    }
-    uint32_t code_next = cl[0];  // codes are filled to tend + 18
-    for (int t = 0; t <= tend; ++t) {
-      const uint32_t code = code_next;
-      code_next = cl[t + 1];
-      double pinj, qinj;
-      if (first) {
-        if (ANCH) {
-          pinj = qinj = t == 0 ? C : 0.0;
-        } else {
-          pinj = (t == 0 || t - 1 > gp.m) ? 0.0 : C;
-          qinj = t == 0 ? 0.0 : C;
-        }
-      } else {
-        if (t > 0 && (t & 15) == 0) {
-          bP = nP;
-          bQ = nQ;
-          nP = nQ = 0.0;
-          if (t + l + 15 <= gp.m_max) {
-            nP = inP[t + l + 32];
-            nQ = inQ[t + l + 32];
-          }
-        }
-        pinj = bP * sc1 * sc2;
-        qinj = bQ * sc1 * sc2;
-        bP = dpp_shl1(0.0, bP);
-        bQ = dpp_shl1(0.0, bQ);
-      }
-      const double Pin = dpp_shr1(pinj, Plast);
-      const double Xn = dpp_shr1(qinj, Q[R - 1]);
-      if (t >= tmain)
-        Plast = adiag_step<double, R, true>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
-      else
-        Plast = adiag_step<double, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
-      Xprev = Xn;
-      if (last) {
-        if (ANCH) {
-          rmax = fmax(rmax, Q[R - 1]);
-          if (t - l - 1 == gp.m) latch = Q[R - 1];
-        } else {
-          acc += Plast;
-        }
-      } else if (l == GROUP - 1) {
-        oP[t + 1] = Plast;  // column t-16 -> slot t+1
-        oQ[t + 1] = Q[R - 1];
-        bmax = fmax(bmax, fmax(Plast, Q[R - 1]));
-      }
-    }
+    // 2^s is applied as two exact factors (s can exceed one factor's exponent
+    // range), once per loaded boundary value
+    double bmax = 0.0;
+#define VD_TILE(F, L)                                                                                               \
+  tile64<ANCH, F, L>(gp, cl, tmain, tend, gd, bd, mu, be, rc, gam, gam_last, xx, sc1, sc2, inP, inQ, oP, oQ, acc,  \
+                     rmax, latch, bmax)
+    if (first && last) VD_TILE(true, true);
+    else if (first) VD_TILE(true, false);
+    else if (last) VD_TILE(false, true);
+    else VD_TILE(false, false);
+#undef VD_TILE
     if (!last) {
-      // renormalise the boundary row to 2^CEXP64; 2^s is applied as two
-      // factors because s can exceed the FP64 exponent range of one factor.
-      // A tile above the group's first read row (a unit mate has more tiles)
-      // holds pass-through rows only and hands its injection on exactly,
-      // unscaled, so the pair's real tiles see the values and renormalisations
-      // of a unit of its own: its result depends on the pair alone.
+      // renormalise the boundary row to 2^CEXP64.  A tile above the group's
+      // first read row (a unit mate has more tiles) holds pass-through rows
+      // only and hands its injection on exactly, unscaled, so the pair's
+      // real tiles see the values and renormalisations of a unit of its own:
+      // its result depends on the pair alone.
       int e = 0;
       if (bmax > 0.0) frexp(bmax, &e);
       const bool real_rows = tile >= ntiles - ntile_g;
@@ -1150,6 +1190,73 @@ __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, cons
     out[gp.out] = v;
   }
   __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 work list binned by shape.  A unit of the FP64 kernel runs as many
+// 64-row tiles as its tallest pair and as many steps as its longest
+// haplotype, so the list is counting-sorted into bins of (tiles, haplotype
+// length / 128), largest first (longest work first for the persistent grid),
+// before the FP64 kernel takes its units in bin order.  A pair's FP64 result
+// does not depend on its unit mates, so the order changes no value.
+
+constexpr int F64_NT = 16;                 // tile-count classes 1 .. 16 (taller reads share the last)
+constexpr int F64_NM = 32;                 // haplotype-length classes of 128 columns
+constexpr int F64_NBIN = F64_NT * F64_NM;  // bins[0 .. NBIN): counts; bins[NBIN .. 2 NBIN): cursors
+static_assert(F64_NBIN == F64_BINS_TOTAL && F64_NBIN <= 1024, "bin layout");
+
+__device__ __forceinline__ int f64_bin(const BatchDev& b, const PairRef& pr, int anchored) {
+  const int np = (int)(b.read_off[pr.read + 1] - b.read_off[pr.read]) + anchored;
+  const int m = (int)(b.hap_off[pr.hap + 1] - b.hap_off[pr.hap]);
+  int nt = (np + GROUP * R64 - 1) / (GROUP * R64);
+  nt = nt < 1 ? 1 : (nt > F64_NT ? F64_NT : nt);
+  int mb = m / 128;
+  mb = mb >= F64_NM ? F64_NM - 1 : mb;
+  return (F64_NT - nt) * F64_NM + (F64_NM - 1 - mb);
+}
+
+__global__ __launch_bounds__(256) void k_f64_hist(BatchDev b, PlanDev p, int anchored,
+                                                  unsigned long long* __restrict__ bins) {
+  __shared__ unsigned int loc[F64_NBIN];
+  for (int i = threadIdx.x; i < F64_NBIN; i += blockDim.x) loc[i] = 0u;
+  __syncthreads();
+  unsigned long long count = p.ctr[CTR_F64];
+  if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    atomicAdd(&loc[f64_bin(b, p.f64_list[i], anchored)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < F64_NBIN; i += blockDim.x)
+    if (loc[i]) atomicAdd(&bins[i], (unsigned long long)loc[i]);
+}
+
+// one block of F64_NBIN threads: the exclusive scan of the counts into the cursors
+__global__ __launch_bounds__(F64_NBIN) void k_f64_scan(unsigned long long* __restrict__ bins) {
+  __shared__ unsigned long long s[F64_NBIN];
+  const int t = threadIdx.x;
+  const unsigned long long own = bins[t];
+  s[t] = own;
+  __syncthreads();
+  for (int d = 1; d < F64_NBIN; d <<= 1) {
+    const unsigned long long v = t >= d ? s[t - d] : 0ull;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  bins[F64_NBIN + t] = s[t] - own;
+}
+
+__global__ __launch_bounds__(256) void k_f64_scatter(BatchDev b, PlanDev p, int anchored,
+                                                     unsigned long long* __restrict__ bins,
+                                                     PairRef* __restrict__ sorted) {
+  unsigned long long count = p.ctr[CTR_F64];
+  if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    const PairRef pr = p.f64_list[i];
+    const unsigned long long pos = atomicAdd(&bins[F64_NBIN + f64_bin(b, pr, anchored)], 1ull);
+    sorted[pos] = pr;
+  }
 }
 
 // 3 waves per SIMD, no spills (158-162 VGPRs with the boundary-row conveyor).
@@ -1317,8 +1424,9 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anc
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                            const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
                            const int* blocks32s,
-                           const int* unroll32, int blocks64, double* ws64, double* out, hipStream_t st0,
-                           hipEvent_t* ev, hipEvent_t* ev_cls, const SideLane* lane2) {
+                           const int* unroll32, int blocks64, double* ws64, PairRef* f64_sorted,
+                           unsigned long long* f64_bins, double* out, hipStream_t st0, hipEvent_t* ev,
+                           hipEvent_t* ev_cls, const SideLane* lane2) {
   hipStream_t st = st0;
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, anchored, force64, out, st);
@@ -1368,10 +1476,18 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
     }
   }
   if (ev) (void)hipEventRecord(ev[2], st);
+  PlanDev p64 = p;
+  if (f64_sorted && f64_bins) {  // bin the FP64 list by shape (fill kernel, then count, scan, scatter)
+    hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(256), 0, st, f64_bins, 2 * F64_NBIN);
+    hipLaunchKernelGGL(k_f64_hist, dim3(1024), dim3(256), 0, st, b, p, anchored, f64_bins);
+    hipLaunchKernelGGL(k_f64_scan, dim3(1), dim3(F64_NBIN), 0, st, f64_bins);
+    hipLaunchKernelGGL(k_f64_scatter, dim3(1024), dim3(256), 0, st, b, p, anchored, f64_bins, f64_sorted);
+    p64.f64_list = f64_sorted;
+  }
   if (anchored)
-    hipLaunchKernelGGL(k_fwd64<true>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
+    hipLaunchKernelGGL(k_fwd64<true>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p64, tab64, smax, ws64, out);
   else
-    hipLaunchKernelGGL(k_fwd64<false>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p, tab64, smax, ws64, out);
+    hipLaunchKernelGGL(k_fwd64<false>, dim3(blocks64), dim3(64), lds_fwd64(smax), st, b, p64, tab64, smax, ws64, out);
   if (ev) (void)hipEventRecord(ev[3], st);
   return hipGetLastError();
 }
