@@ -886,7 +886,7 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
 // ---- diploid genotype likelihood ------------------------------------------
 
 struct GenoBufs {
-  DevBuf mol, keep, units, map, ctr, mq_err, kidx, nkept;
+  DevBuf mol, keep, units, map, ctr, mq_err, kidx, nkept, status;
   DevBuf in_off, in_lh, in_gl, in_mq, in_vals, in_cert;  // host entry points
   bool mq_err_ready = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -936,7 +936,7 @@ vd::GenoDev geno_dev(const vd_geno_batch* b) {
   d.nrows = b->nrows;
   d.mat_size = b->mat_size;
   d.gl_size = b->gl_size;
-  d.unit_cap = b->nregions + b->gl_size / 64 + 1;  // sum of ceil(pairs/64) when blocks do not overlap
+  d.unit_cap = b->gl_size / 64 + 2;  // ceil(sum of pairs / 64) when blocks do not overlap
   return d;
 }
 
@@ -965,6 +965,19 @@ int geno_stats(hipStream_t st, vd_geno_stats* s, bool molecule) {
   return 0;
 }
 
+// the mixture's workspaces (sized by the batch; see vd::MixScratch)
+int mix_scratch(const vd_geno_batch* b, const vd::GenoDev& d, vd::MixScratch* w) {
+  const int64_t nreg = std::max<int64_t>(b->nregions, 1);
+  HIPCHK(gg.units.ensure(sizeof(int64_t) * 2 * (b->nregions + 1)));
+  HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
+  HIPCHK(gg.kidx.ensure(sizeof(int32_t) * std::max<int64_t>(b->nrows, 1)));
+  HIPCHK(gg.nkept.ensure(sizeof(int32_t) * nreg));
+  HIPCHK(gg.status.ensure(nreg));
+  *w = vd::MixScratch{gg.units.as<int64_t>(), gg.map.as<int32_t>(), gg.kidx.as<int32_t>(), gg.nkept.as<int32_t>(),
+                      gg.status.as<int8_t>()};
+  return 0;
+}
+
 int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t* keep, double prior, int tri,
                           double* gl, hipStream_t st, vd_geno_stats* s) {
   if (int rc = check_geno_batch(b)) return rc;
@@ -975,15 +988,11 @@ int mixture_device_locked(const vd_geno_batch* b, const double* lh, const int8_t
   StreamOrder order(st);
   if (int rc = geno_init(st)) return rc;
   const vd::GenoDev d = geno_dev(b);
-  HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
-  HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
+  vd::MixScratch w;
+  if (int rc = mix_scratch(b, d, &w)) return rc;
   if (int rc = geno_zero_ctr(st)) return rc;
   if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
-  HIPCHK(gg.kidx.ensure(sizeof(int32_t) * std::max<int64_t>(b->nrows, 1)));
-  HIPCHK(gg.nkept.ensure(sizeof(int32_t) * std::max<int64_t>(b->nregions, 1)));
-  HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, gg.units.as<int64_t>(), gg.map.as<int32_t>(),
-                            gg.kidx.as<int32_t>(), gg.nkept.as<int32_t>(), gl, gg.ctr.as<unsigned long long>(),
-                            g.cus, st));
+  HIPCHK(vd::launch_mixture(d, lh, keep, prior, tri, w, gl, gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
   return geno_stats(st, s, false);
 }
@@ -1002,17 +1011,14 @@ int genotype_device_locked(const vd_geno_batch* b, const double* ll, const uint8
   const vd::GenoDev d = geno_dev(b);
   HIPCHK(gg.mol.ensure(sizeof(double) * std::max<int64_t>(b->mat_size, 1)));
   HIPCHK(gg.keep.ensure(std::max<int64_t>(b->nrows, 1)));
-  HIPCHK(gg.units.ensure(sizeof(int64_t) * (b->nregions + 1)));
-  HIPCHK(gg.map.ensure(sizeof(int32_t) * d.unit_cap));
+  vd::MixScratch w;
+  if (int rc = mix_scratch(b, d, &w)) return rc;
   if (int rc = geno_zero_ctr(st)) return rc;
   if (s) HIPCHK(hipEventRecord(gg.ev[0], st));
   HIPCHK(vd::launch_molecule(d, ll, mq, gg.mq_err.as<double>(), min_cert, values, cert, gg.mol.as<double>(),
                              gg.keep.as<int8_t>(), gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[1], st));
-  HIPCHK(gg.kidx.ensure(sizeof(int32_t) * std::max<int64_t>(b->nrows, 1)));
-  HIPCHK(gg.nkept.ensure(sizeof(int32_t) * std::max<int64_t>(b->nregions, 1)));
-  HIPCHK(vd::launch_mixture(d, gg.mol.as<double>(), gg.keep.as<int8_t>(), prior, 1, gg.units.as<int64_t>(),
-                            gg.map.as<int32_t>(), gg.kidx.as<int32_t>(), gg.nkept.as<int32_t>(), gl,
+  HIPCHK(vd::launch_mixture(d, gg.mol.as<double>(), gg.keep.as<int8_t>(), prior, 1, w, gl,
                             gg.ctr.as<unsigned long long>(), g.cus, st));
   if (s) HIPCHK(hipEventRecord(gg.ev[2], st));
   return geno_stats(st, s, true);

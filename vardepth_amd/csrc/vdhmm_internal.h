@@ -145,7 +145,7 @@ struct GenoDev {
   const int64_t* gl_off;
   int64_t nregions, nrows;
   int64_t mat_size, gl_size;  // entries of the matrices / of the pair outputs (bounds of every block)
-  int64_t unit_cap;           // capacity of the wave-unit map
+  int64_t unit_cap;           // capacity of each wave-unit map (units of 64 items)
 };
 
 enum {
@@ -154,6 +154,7 @@ enum {
   GCTR_KEPT = 2,     // rows that passed the certainty filter
   GCTR_LAYOUT = 3,   // regions skipped: block outside the matrix / output bounds, or unit map full
   GCTR_GUARD = 4,    // regions flagged by the near-tie guard
+  GCTR_UNITS_WIDE = 5,  // k_mix_wide wave units (written by the scan)
   GCTR_COUNT = 8
 };
 
@@ -212,7 +213,15 @@ hipError_t launch_guard(const GenoDev& g, const double* gl, const double* cert, 
 hipError_t launch_molecule(const GenoDev& g, const double* ll, const uint8_t* mq, const double* mq_err,
                            double min_cert, double* values, double* cert, double* mol, int8_t* keep,
                            unsigned long long* ctr, int cus, hipStream_t st);
+// Workspaces of the mixture: cnt 2 x (nregions + 1) int64; unit_first
+// unit_cap int32; kidx nrows int32; nkept nregions int32; status nregions int8.
+struct MixScratch {
+  int64_t* cnt;
+  int32_t* unit_first;
+  int32_t* kidx;
+  int32_t* nkept;
+  int8_t* status;
+};
 hipError_t launch_mixture(const GenoDev& g, const double* mat, const int8_t* keep, double prior, int triangular,
-                          int64_t* unit_off, int32_t* unit_region, int32_t* kidx, int32_t* nkept, double* gl,
-                          unsigned long long* ctr, int cus, hipStream_t st);
+                          const MixScratch& w, double* gl, unsigned long long* ctr, int cus, hipStream_t st);
 }  // namespace vd

@@ -1251,11 +1251,31 @@ __global__ __launch_bounds__(256) void k_f64_scatter(BatchDev b, PlanDev p, int 
                                                      PairRef* __restrict__ sorted) {
   unsigned long long count = p.ctr[CTR_F64];
   if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    const PairRef pr = p.f64_list[i];
-    const unsigned long long pos = atomicAdd(&bins[F64_NBIN + f64_bin(b, pr, anchored)], 1ull);
-    sorted[pos] = pr;
+  const int lane = threadIdx.x & 63;
+  // one cursor atomic per distinct bin of a wave: the list is dominated by a
+  // few bins, where one atomic per entry serialises on the same addresses
+  for (unsigned long long w = (unsigned long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); w < count;
+       w += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long i = w + lane;
+    const bool act = i < count;
+    PairRef pr{};
+    int bin = -1;
+    if (act) {
+      pr = p.f64_list[i];
+      bin = f64_bin(b, pr, anchored);
+    }
+    unsigned long long pending = __builtin_amdgcn_ballot_w64(act), pos = 0;
+    while (pending) {
+      const int leader = __builtin_ctzll(pending);
+      const int lb = __shfl(bin, leader);
+      const unsigned long long same = __builtin_amdgcn_ballot_w64(bin == lb);
+      unsigned long long start = 0;
+      if (lane == leader) start = atomicAdd(&bins[F64_NBIN + lb], (unsigned long long)__builtin_popcountll(same));
+      start = __shfl(start, leader);
+      if (bin == lb) pos = start + (unsigned long long)__builtin_popcountll(same & ((1ull << lane) - 1ull));
+      pending &= ~same;
+    }
+    if (act) sorted[pos] = pr;
   }
 }
 
