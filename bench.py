@@ -248,6 +248,19 @@ def plain_baseline(batch, mode: int, pairs: int):
             "kind": "one kernel launch per pair, one thread per read row, FP32, inputs resident in HBM"}
 
 
+def pair_lengths(batch):
+    """Read and haplotype length of every pair, in the batch's output order
+    (per region, row-major reads x haplotypes)."""
+    rl, hl = batch.read_lengths(), batch.hap_lengths()
+    ns, ms = [], []
+    for k in range(batch.nregions):
+        r = rl[batch.region_reads[k]:batch.region_reads[k + 1]]
+        h = hl[batch.region_haps[k]:batch.region_haps[k + 1]]
+        ns.append(np.repeat(r, len(h)))
+        ms.append(np.tile(h, len(r)))
+    return (np.concatenate(ns) if ns else np.zeros(0, np.int64)), (np.concatenate(ms) if ms else np.zeros(0, np.int64))
+
+
 def accuracy_configs(mode: int):
     """The single 64x8 region and the 1024-region precision-stress workload:
     GCUPS, FP32 pairs rescued in FP64, and per-pair |log10 L - log10 L_FP64|
@@ -279,6 +292,10 @@ def accuracy_configs(mode: int):
     f3 = dev.result()
     ok = np.isfinite(f3)
     e3 = np.abs(m3[ok] - f3[ok]) / ln10
+    # the reference's own rescue rule for its mixed GPU path (accelerator.py:139-154):
+    # FP64 on the host when n > 512, m > 1024, the result is not finite or |ln L| > 80
+    pn, pm = pair_lengths(b3)
+    ref_rule = (pn > 512) | (pm > 1024) | ~np.isfinite(f3) | (np.abs(f3) > 80)
     sub = b3.subset([0, 1])
     ref3, _ = oracle.regions_cpu(sub, mode=mode, threads=4)
     n_sub = sub.npairs
@@ -289,6 +306,8 @@ def accuracy_configs(mode: int):
         "pairs": b3.npairs, "cells": b3.cells(), "fwd_ms": round(st3["device_ms"], 3),
         "gcups": round(b3.cells() / st3["device_ms"] / 1e6, 2), "fp64_all_ms": round(st64["device_ms"], 3),
         "flagged": int(st3["flagged"]), "flagged_frac": round(st3["flagged"] / b3.npairs, 4),
+        "reference_rule_frac": round(float(ref_rule.mean()), 4),
+        "reference_rule_abs_ln_gt_80_frac": round(float((np.abs(f3) > 80).mean()), 4),
         "max_log10_err_vs_fp64": float(e3.max()), "p99_log10_err_vs_fp64": float(np.quantile(e3, 0.99)),
         "fp64_vs_cpu_oracle_max_log10": float(e_or.max()), "oracle_sample_pairs": int(n_sub)}
     return out

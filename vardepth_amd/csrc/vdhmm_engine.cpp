@@ -140,7 +140,7 @@ struct Engine {
   // staged inputs / outputs of the host entry points
   DevBuf in_read, in_qual, in_roff, in_hap, in_hoff, in_rreg, in_hreg, in_ooff, in_out;
   // plan scratch
-  DevBuf read_cls, read_exp, class_off, read_perm, unit_cnt, unit_bucket, f64_list, ctr, ws64;
+  DevBuf read_cls, read_exp, read_ln32, class_off, read_perm, unit_cnt, unit_bucket, f64_list, ctr, ws64;
   DevBuf f64_sorted, f64_bins;  // the FP64 list binned by shape, and its bin counters
   // parameter tables
   DevBuf qE, tab32, tab64, gtab;
@@ -231,23 +231,35 @@ int check_hap_len(int max_hap_len, int gradients) {
   return 0;
 }
 
+// Normaliser rho_q of quality bin q (the row constants' comment in
+// vdhmm_internal.h): for FP64 the power of two 2^E >= max(1-e, e/3)(1-2g)
+// (E is returned either way), for FP32 the largest scaled row factor itself.
+double normaliser(const vd_params& p, int q, bool fp32, int* E) {
+  const double a = 1 - 2 * p.gap, b = 1 - p.ext, c = 0.25 * p.gap, d = 0.25 * p.ext;
+  const double e = p.err[q];
+  const double best = std::max(1 - e, e / 3) * a;  // >= a/4 > 0
+  int ex = 0;
+  std::frexp(best, &ex);  // best = f 2^ex, f in [0.5, 1)
+  if (E) *E = ex;
+  if (!fp32) return std::ldexp(1.0, ex);
+  return std::max(std::max(best, d), std::max(c, b * c / a));
+}
+
 template <typename T>
 void build_tab(const vd_params& p, vd::QTab<T>& t, int32_t* E) {
   const double a = 1 - 2 * p.gap, b = 1 - p.ext, c = 0.25 * p.gap, d = 0.25 * p.ext;
   for (int q = 0; q < 64; ++q) {
     const double e = p.err[q];
     const double gd = 1 - e, bd = e / 3;
-    const double best = std::max(gd, bd) * a;
     int ex = 0;
-    if (best > 0) std::frexp(best, &ex);  // best = f 2^ex, f in [0.5, 1)
-    const double rho = std::ldexp(1.0, ex);
+    const double rho = normaliser(p, q, sizeof(T) == 4, &ex);
     t.good[q] = (T)(gd * a / rho);
     t.bad[q] = (T)(bd * a / rho);
     t.mu[q] = (T)(d / rho);
     t.beta[q] = (T)((b * c / a) / rho);
     t.cfin[q] = (T)(c / rho);
-    t.E[q] = ex;
-    E[q] = ex;
+    t.lnrho[q] = std::log(rho);
+    E[q] = ex;  // the FP64 exponent, whichever precision's table this is
   }
   t.gamma = (T)(b * p.gap / a);
   t.gap = (T)p.gap;
@@ -333,6 +345,7 @@ int plan_buffers(const vd_batch* db, int64_t npairs) {
   const int64_t max_units = npairs / vd::PAIRS_PER_UNIT + nb + 1;
   HIPCHK(g.read_cls.ensure(nreads));
   HIPCHK(g.read_exp.ensure(nreads * 4));
+  HIPCHK(g.read_ln32.ensure(nreads * 8));
   HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
   HIPCHK(g.read_perm.ensure(nreads * 8));
   HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
@@ -348,7 +361,8 @@ vd::BatchDev batch_dev(const vd_batch* db) {
 }
 
 vd::PlanDev plan_dev(int64_t npairs, int class_major, int trie = 0) {
-  return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.class_off.as<int32_t>(),
+  return vd::PlanDev{g.read_cls.as<int8_t>(), g.read_exp.as<int32_t>(), g.read_ln32.as<double>(),
+                     g.class_off.as<int32_t>(),
                      g.read_perm.as<int64_t>(), g.unit_cnt.as<int64_t>(), g.unit_bucket.as<int32_t>(),
                      g.f64_list.as<vd::PairRef>(), g.ctr.as<unsigned long long>(), std::max<int64_t>(npairs, 1),
                      class_major, trie, g.tune.share_chunk, g.trie_order.as<int32_t>(), g.trie_start.as<int32_t>(),
@@ -521,13 +535,14 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   };
   const size_t o_qE = slice(64 * sizeof(int32_t)), o_t32 = slice(sizeof(vd::QTab<float>)),
                o_t64 = slice(sizeof(vd::QTab<double>)), o_cls = slice(nreads), o_exp = slice(nreads * 4),
+               o_ln32 = slice(nreads * 8),
                o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8), o_ucnt = slice((nb + 1) * 8),
                o_ubkt = slice(max_units * 4), o_f64 = slice(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)),
                o_ctr = slice(vd::CTR_COUNT * 8);
   const size_t ws_block = (size_t)4 * vd::PAIRS_PER_UNIT * smax * sizeof(double);
   if (off + ws_block > cap)
     return fail("vdhmm: this batch needs " + std::to_string(off + ws_block) +
-                " bytes of scratch (13 per read, 17 per pair, 204 per region and one FP64 tile workspace of 128 per "
+                " bytes of scratch (21 per read, 17 per pair, 204 per region and one FP64 tile workspace of 128 per "
                 "haplotype slot); the no-allocation forward holds " + std::to_string(cap));
   if (g.occ_smax != smax) {
     for (int a = 0; a < 2; ++a) {
@@ -564,6 +579,7 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   vd::PlanDev p{};
   p.read_cls = reinterpret_cast<int8_t*>(B + o_cls);
   p.read_exp = reinterpret_cast<int32_t*>(B + o_exp);
+  p.read_ln32 = reinterpret_cast<double*>(B + o_ln32);
   p.class_off = reinterpret_cast<int32_t*>(B + o_coff);
   p.read_perm = reinterpret_cast<int64_t*>(B + o_perm);
   p.unit_cnt = reinterpret_cast<int64_t*>(B + o_ucnt);
@@ -673,10 +689,8 @@ void build_grad_tab(const vd_params& p, vd::GradTab& t) {
   const double g = p.gap, x = p.ext, a = 1 - 2 * g;
   for (int q = 0; q < 64; ++q) {
     const double e = p.err[q];
-    const double best = std::max(1 - e, e / 3) * a;
-    int ex = 0;
-    if (best > 0) std::frexp(best, &ex);
-    const double rho = std::ldexp(1.0, ex);
+    const double rho = normaliser(p, q, false, nullptr);
+    t.dgood_de32[q] = -a / normaliser(p, q, true, nullptr);
     t.dgood_dg[q] = -2 * (1 - e) / rho;
     t.dbad_dg[q] = -2 * (e / 3) / rho;
     t.dgood_de[q] = -a / rho;
@@ -842,9 +856,9 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
   const int64_t todo = std::min<int64_t>(std::max<int64_t>(max_pairs, 0), npairs);
   std::vector<double> lnk(hb->nreads);
   for (int64_t r = 0; r < hb->nreads; ++r) {
-    int64_t e = 0;
-    for (int64_t k = hb->read_off[r]; k < hb->read_off[r + 1]; ++k) e += E[std::min<int>(hb->qual_bytes[k], 63)];
-    lnk[r] = (double)(e - vd::CEXP32) * 0.69314718055994530942;
+    double e = 0;
+    for (int64_t k = hb->read_off[r]; k < hb->read_off[r + 1]; ++k) e += t32.lnrho[std::min<int>(hb->qual_bytes[k], 63)];
+    lnk[r] = e - vd::CEXP32 * 0.69314718055994530942;
   }
   const uint8_t* dread = g.in_read.as<uint8_t>();
   const uint8_t* dqual = g.in_qual.as<uint8_t>();

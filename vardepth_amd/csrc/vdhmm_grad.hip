@@ -188,7 +188,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   bool flag = false;
   double seed = 0.0, wpair = 0.0;
   if (l == GROUP - 1 && gp.valid) {
-    const double lnk = (double)(p.read_exp[gp.read] - GradScale<T>::CEXP) * LN2;
+    const double lnk = sizeof(T) == 4 ? p.read_ln32[gp.read] - GradScale<T>::CEXP * LN2
+                                      : (double)(p.read_exp[gp.read] - GradScale<T>::CEXP) * LN2;
     double v;
     if (ANCH) {
       const double lc = log((double)rmax) + LN_1EM300;
@@ -365,7 +366,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
       if (kk >= 0) {
         int q = b.qual_bytes[rb + kk];
         q = q < 63 ? q : 63;
-        atomicAdd(&lds_de[q], (double)Er[r] * gt->dgood_de[q] * -1.0);  // dgood_de = -a/rho
+        // d good/de = -a/rho for the precision's own normaliser
+        atomicAdd(&lds_de[q], (double)Er[r] * (sizeof(T) == 4 ? gt->dgood_de32[q] : gt->dgood_de[q]) * -1.0);
       }
     }
     const double u = (double)U + (last_lane ? 0.0 : (double)UL);
@@ -418,11 +420,11 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
   const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   if (lo >= hi) return;  // no read of this class: the block's partial row stays as it is
   extern __shared__ __align__(16) unsigned char lds[];
-  grad_lds_init<float>(gtab, sizeof(QTab<float>), lds);
+  grad_lds_init<float>(gtab, qtab_lds_bytes<float>(), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
   const QTab<float>* tab = reinterpret_cast<const QTab<float>*>(lds + 64 * sizeof(double));
   uint16_t* codes =
-      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
+      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)));
   float* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63;
   double acc[2] = {0.0, 0.0};
@@ -507,11 +509,11 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
                                                const double* __restrict__ weights, double* __restrict__ out,
                                                double* __restrict__ gpart) {
   extern __shared__ __align__(16) unsigned char lds[];
-  grad_lds_init<double>(gtab, sizeof(QTab<double>), lds);
+  grad_lds_init<double>(gtab, qtab_lds_bytes<double>(), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
   const QTab<double>* tab = reinterpret_cast<const QTab<double>*>(lds + 64 * sizeof(double));
   uint16_t* codes =
-      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((sizeof(QTab<double>) + 15) & ~size_t(15)));
+      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
   double* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63, g = lane >> 4;
   constexpr int BIN = R / 2 - 1;
@@ -574,11 +576,11 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const double* __restrict__ 
 }
 
 size_t lds_grad32(int smax) {
-  return 64 * sizeof(double) + ((sizeof(QTab<float>) + 15) & ~size_t(15)) +
+  return 64 * sizeof(double) + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)) +
          (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 size_t lds_grad64(int smax) {
-  return 64 * sizeof(double) + ((sizeof(QTab<double>) + 15) & ~size_t(15)) +
+  return 64 * sizeof(double) + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)) +
          (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
@@ -660,7 +662,7 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
                        double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls) {
   if (ev) (void)hipEventRecord(ev[0], st);
-  launch_plan(b, p, qE, anchored, 0, out, st);
+  launch_plan(b, p, qE, tab32, anchored, 0, out, st);
   hipLaunchKernelGGL(k_zero_f64, dim3(64), dim3(256), 0, st, gpart, (int64_t)max_blocks * NGRAD);
   if (ev) (void)hipEventRecord(ev[1], st);
   const size_t s32 = grad_ck_stride(smax, 0), s64 = grad_ck_stride(smax, 1);

@@ -2,6 +2,7 @@
 // engine (vdhmm_engine.cpp).  Not part of the public C-ABI (include/vdhmm.h).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace vd {
@@ -36,27 +37,42 @@ constexpr int NGRAD = 66;     // gradient layout: [0] d/dgap, [1] d/dext, [2 + q
 
 // Per-quality-bin row constants of the rescaled recurrence, for one
 // precision.  With a = 1-2g, b = 1-x, c = g/4, d = x/4 and a per-bin
-// power-of-two normaliser rho_q = 2^E_q >= max(1-e, e/3) * a:
+// normaliser rho_q:
 //   good = (1-e) a / rho, bad = (e/3) a / rho     (match emission, scaled)
 //   mu   = d / rho                                 (I recurrence of the next row)
 //   beta = (b c / a) / rho                         (I term of the next row's M)
 //   cfin = c / rho                                 (terminal weight of I)
 //   gamma = b g / a                                (D term of the next row's M)
+// FP64: rho_q = 2^E_q >= max(1-e, e/3) a, a power of two, so that the FP64
+// results are exact rescalings (the per-read sum of E_q is an integer).
+// FP32: rho_q = max((1-e) a, (e/3) a, d, c, b c / a) exactly, so that the
+// largest per-row factor is 1 and a matching row costs no FP32 range (a
+// power of two would cost up to one bit per row); the per-read sum of ln rho
+// is taken in FP64.  Either way every row factor is at most 1, and the
+// constants are rounded once, when the table is built.
 template <typename T>
 struct QTab {
   T good[64], bad[64], mu[64], beta[64], cfin[64];
-  int E[64];
   T gamma, gap, ext;
+  double lnrho[64];  // ln rho_q: read by the planner only, so kernels stage the fields before it
 };
+// Bytes of a table that kernels stage in LDS (every field before lnrho).
+template <typename T>
+constexpr size_t qtab_lds_bytes() {
+  return offsetof(QTab<T>, lnrho);
+}
 
 // Partial derivatives of the scaled row constants with respect to g, x and
-// e_q (rho_q held fixed: it is a power of two chosen per bin), used to turn
-// the adjoint sums into gradient terms:
+// e_q with rho_q held fixed (the likelihood is the same for any fixed choice
+// of normalisers, which the log correction cancels, so the derivative at
+// fixed rho is the derivative), used to turn the adjoint sums into gradient
+// terms:
 //   d good/dg = -2(1-e)/rho  d bad/dg = -2(e/3)/rho  d good/de = -a/rho  d bad/de = a/(3 rho)
 //   d mu/dx = 1/(4 rho)      d beta/dx = -g/(4 a rho) d beta/dg = (1-x)/(4 rho a^2)
 //   d cfin/dg = 1/(4 rho)    d gamma/dg = (1-x)/a^2   d gamma/dx = -g/a
 struct GradTab {
   double dgood_dg[64], dbad_dg[64], dgood_de[64], dbad_de[64];
+  double dgood_de32[64];  // d good/de with the FP32 normaliser (the others hold the FP64 one)
   double dmu_dx[64], dbeta_dx[64], dbeta_dg[64], dcfin_dg[64];
   double dgam_dg, dgam_dx;
 };
@@ -82,7 +98,8 @@ struct BatchDev {
 // Device scratch of one planned batch.
 struct PlanDev {
   int8_t* read_cls;         // [nreads]
-  int32_t* read_exp;        // [nreads]  sum of E_q over the read
+  int32_t* read_exp;        // [nreads]  sum of E_q over the read (FP64 normalisers)
+  double* read_ln32;        // [nreads]  sum of ln rho_q over the read (FP32 normalisers)
   int32_t* class_off;       // [nregions * (NCLS+1)] per-region class offsets
   int64_t* read_perm;       // [nreads]  reads grouped by class inside each region
   int64_t* unit_cnt;        // [nregions * NCLS + 1] FP32 units per bucket, scanned in place
@@ -171,8 +188,8 @@ size_t lds_fwd32(int smax);
 size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
 hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
-void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
-                 hipStream_t st);  // p.class_major selects the bucket order
+void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32, int anchored,
+                 int force64, double* out, hipStream_t st);  // p.class_major selects the bucket order
 // A second stream for the FP32 class kernels: consecutive classes alternate
 // between the caller's stream and `side`, so that the last units of one
 // persistent kernel overlap the first of the next.  Kernels on `side` use the

@@ -39,28 +39,38 @@ namespace vd {
 // ---------------------------------------------------------------------------
 // planning
 
-// Class and exponent sum of every read: one wave per read, coalesced loads
-// of the quality bytes, wave reduction of the per-bin exponents.
-__global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, int anchored, int force64,
-                             int8_t* __restrict__ cls, int32_t* __restrict__ exps) {
+// Class and normaliser sums of every read: one wave per read, coalesced
+// loads of the quality bytes, wave reductions of the per-bin FP64 exponents
+// and of the FP32 ln rho (lane partial sums in a fixed order, then a fixed
+// butterfly, so the FP64 sum of a read does not depend on the batch).
+__global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, const QTab<float>* __restrict__ tab32,
+                             int anchored, int force64, int8_t* __restrict__ cls, int32_t* __restrict__ exps,
+                             double* __restrict__ ln32) {
   const int lane = threadIdx.x & 63;
   const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   for (int64_t r = wave0; r < b.nreads; r += nwaves) {
     const int64_t o0 = b.read_off[r], o1 = b.read_off[r + 1];
     int e = 0;
+    double l = 0.0;
     for (int64_t k = o0 + lane; k < o1; k += 64) {
       int q = b.qual_bytes[k];
-      e += qE[q < 63 ? q : 63];
+      q = q < 63 ? q : 63;
+      e += qE[q];
+      l += tab32->lnrho[q];
     }
 #pragma unroll
-    for (int off = 32; off; off >>= 1) e += __shfl_xor(e, off);
+    for (int off = 32; off; off >>= 1) {
+      e += __shfl_xor(e, off);
+      l += __shfl_xor(l, off);
+    }
     if (lane == 0) {
       const int n = (int)(o1 - o0);
       const int np = n + anchored;
       const int c = n == 0 ? CLS_EMPTY : ((force64 || np > GROUP * MAX_R32) ? CLS_F64 : (np + GROUP - 1) / GROUP);
       cls[r] = (int8_t)c;
       exps[r] = e;
+      ln32[r] = l;
     }
   }
 }
@@ -463,7 +473,7 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
   if (t <= tend) tail_step(t++);
   count_groups(&p.ctr[CTR_SWEPT], gp.valid ? (unsigned long long)gp.n * (unsigned long long)gp.m : 0ull);
   if (l == GROUP - 1 && gp.valid) {
-    const double lnk = (double)(p.read_exp[gp.read] - CEXP32) * LN2;
+    const double lnk = p.read_ln32[gp.read] - CEXP32 * LN2;
     double v;
     bool flag;
     if (ANCH) {
@@ -844,7 +854,7 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
     }
     count_groups(&p.ctr[CTR_SWEPT], rt.valid ? (unsigned long long)rt.n * (unsigned long long)(J - st) : 0ull);
     if (l == GROUP - 1 && rt.valid) {
-      const double lnk = (double)(p.read_exp[rt.read] - CEXP32) * LN2;
+      const double lnk = p.read_ln32[rt.read] - CEXP32 * LN2;
       const int64_t oidx = b.out_off[k] + (rt.read - r0) * H + (hap - h0);
       double v;
       bool flag;
@@ -916,11 +926,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_p(ANCH, 
   if (lo >= hi) return;  // no read of this class in the batch
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds);
-  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)));
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
     uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
-    for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(qtab_lds_bytes<float>() / 4); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -951,11 +961,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, 
   if (lo >= hi) return;
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<float>* tab = reinterpret_cast<QTab<float>*>(lds);
-  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<float>) + 15) & ~size_t(15)));
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)));
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
     uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
-    for (int i = threadIdx.x; i < (int)(sizeof(QTab<float>) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(qtab_lds_bytes<float>() / 4); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -1288,11 +1298,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                                               int smax, double* __restrict__ wsall, double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
-  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((sizeof(QTab<double>) + 15) & ~size_t(15)));
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
     uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
-    for (int i = threadIdx.x; i < (int)(sizeof(QTab<double>) / 4); i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < (int)(qtab_lds_bytes<double>() / 4); i += blockDim.x) dst[i] = src[i];
   }
   __syncthreads();
   double* ws = wsall + (size_t)blockIdx.x * 4 * PAIRS_PER_UNIT * smax;
@@ -1346,10 +1356,10 @@ __global__ void k_zero_ctr(unsigned long long* ctr, int n) {
 }
 
 size_t lds_fwd32(int smax) {
-  return ((sizeof(QTab<float>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
+  return ((qtab_lds_bytes<float>() + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 size_t lds_fwd64(int smax) {
-  return ((sizeof(QTab<double>) + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
+  return ((qtab_lds_bytes<double>() + 15) & ~size_t(15)) + (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
 }
 
 template <bool ANCH, int U>
@@ -1422,14 +1432,15 @@ static void launch_fwd32t_r(int r, int blocks, size_t lds, hipStream_t st, const
   }
 }
 
-void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, int anchored, int force64, double* out,
-                 hipStream_t st) {
+void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32, int anchored,
+                 int force64, double* out, hipStream_t st) {
   const int64_t nb = b.nregions * NCLS;
   hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, p.ctr, (int)CTR_COUNT);
   if (b.nreads > 0) {
     int g = (int)((b.nreads + 3) / 4);  // 4 waves of 64 per block, one read per wave
     g = g > 8192 ? 8192 : g;
-    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, anchored, force64, p.read_cls, p.read_exp);
+    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, tab32, anchored, force64, p.read_cls,
+                       p.read_exp, p.read_ln32);
   }
   if (b.nregions > 0) {
     int g = (int)(b.nregions > 65535 ? 65535 : b.nregions);
@@ -1449,7 +1460,7 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
                            hipEvent_t* ev_cls, const SideLane* lane2) {
   hipStream_t st = st0;
   if (ev) (void)hipEventRecord(ev[0], st);
-  launch_plan(b, p, qE, anchored, force64, out, st);
+  launch_plan(b, p, qE, tab32, anchored, force64, out, st);
   if (ev) (void)hipEventRecord(ev[1], st);
   if (!force64) {
     // per-class timing keeps one stream, so that each class is timed alone
