@@ -290,6 +290,19 @@ def accuracy_configs(mode: int):
     m3 = dev.result()
     st64 = dev.forward(mode=mode, precision="float64", stats=True)
     f3 = dev.result()
+    # the autotuner on this batch (launch choices and FP64 routing), then the
+    # process's previous choices back for the legs that follow
+    from vardepth_amd import autotune
+    saved = autotune.get()
+    try:
+        tune3 = autotune.tune_forward(dev, mode=mode)
+        st3t = min((dev.forward(mode=mode, precision="mixed", stats=True) for _ in range(3)),
+                   key=lambda s_: s_["device_ms"])
+        t3 = dev.result()
+    finally:
+        autotune.set_tuning(saved)
+    okt = np.isfinite(f3)
+    e3t = np.abs(t3[okt] - f3[okt]) / ln10
     ok = np.isfinite(f3)
     e3 = np.abs(m3[ok] - f3[ok]) / ln10
     # the reference's own rescue rule for its mixed GPU path (accelerator.py:139-154):
@@ -307,6 +320,9 @@ def accuracy_configs(mode: int):
         "gcups": round(b3.cells() / st3["device_ms"] / 1e6, 2), "fp64_all_ms": round(st64["device_ms"], 3),
         "flagged": int(st3["flagged"]), "flagged_frac": round(st3["flagged"] / b3.npairs, 4),
         "reference_rule_frac": round(float(ref_rule.mean()), 4),
+        "tuned_ms": round(st3t["device_ms"], 3), "tuned_gcups": round(b3.cells() / st3t["device_ms"] / 1e6, 2),
+        "tuned_route64_classes": tune3.get("route64", {}).get("classes", []),
+        "tuned_max_log10_err_vs_fp64": float(e3t.max()),
         "reference_rule_abs_ln_gt_80_frac": round(float((np.abs(f3) > 80).mean()), 4),
         "max_log10_err_vs_fp64": float(e3.max()), "p99_log10_err_vs_fp64": float(np.quantile(e3, 0.99)),
         "fp64_vs_cpu_oracle_max_log10": float(e_or.max()), "oracle_sample_pairs": int(n_sub)}

@@ -204,10 +204,10 @@ int vd_grad_device(const vd_batch *dev_batch, const vd_params *params, int mode,
  *                       while vd_prefix_sharing is on (the suffix part while
  *                       vd_suffix_sharing is on)
  *   share_chunk         haplotypes per read unit under prefix sharing (1..64)
- * Defaults: 2, 0, 0, 0, 2, 4, 1 (0 with VARDEPTH_CLASS_OVERLAP=0).  Values
- * never change results beyond rounding
- * order (FP64-rescued pairs may differ in the last bit; the suffix
- * continuation agrees with full sweeps to FP32 rounding). */
+ * Defaults: 2, 0, 0, 0, 2, 4, 1 (0 with VARDEPTH_CLASS_OVERLAP=0), 0.  Values
+ * never change results beyond rounding order (the suffix continuation agrees
+ * with full sweeps to FP32 rounding), except fwd_route64, which gives the
+ * routed pairs their FP64 values. */
 typedef struct vd_tuning {
   int fwd_unroll[17];
   int fwd_blocks_per_cu[17];
@@ -217,6 +217,9 @@ typedef struct vd_tuning {
   int share_chunk;
   int class_overlap;  /* 1: consecutive FP32 class kernels alternate between two streams
                          (vd_class_overlap); 0: one stream */
+  int fwd_route64[17];  /* 1: with precision VD_PREC_MIXED, the reads of this class skip
+                           the FP32 pass and go straight to the FP64 rescue (for
+                           batches where FP32 would flag most of them) */
 } vd_tuning;
 
 int vd_tuning_get(vd_tuning *t);

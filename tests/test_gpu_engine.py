@@ -86,6 +86,44 @@ def test_precision_stress_rescue():
     print("stress: pairs %d flagged %d max err %.3g" % (st["pairs"], st["flagged"], err))
 
 
+def test_fp64_routing_gives_fp64_values():
+    """Classes routed straight to FP64 (vd_tuning.fwd_route64) return the
+    all-FP64 values for their pairs, and the other classes their FP32 values;
+    the autotuner keeps routing only for a batch that FP32 mostly flags."""
+    from vardepth_amd import autotune
+    base = autotune.get()
+    b = workloads.uneven_regions(seed=5, regions=24)
+    dev = engine.DeviceBatch(b)
+    try:
+        dev.forward(mode=0, precision="mixed")
+        mixed = dev.result()
+        dev.forward(mode=0, precision="float64")
+        f64 = dev.result()
+        rl = b.read_lengths()
+        cls = (rl + 15) // 16  # rows per lane of each read (mode 0)
+        routed = [r for r in range(1, 17) if r % 2 == 1]
+        t = autotune.get()
+        for r in routed:
+            t.fwd_route64[r] = 1
+        autotune.set_tuning(t)
+        dev.forward(mode=0, precision="mixed")
+        out = dev.result()
+        oo = b.out_off()
+        for k in range(b.nregions):
+            r0, r1 = b.region_reads[k], b.region_reads[k + 1]
+            nh = b.region_haps[k + 1] - b.region_haps[k]
+            for i in range(r0, r1):
+                sl = slice(oo[k] + (i - r0) * nh, oo[k] + (i - r0 + 1) * nh)
+                want = f64[sl] if cls[i] in routed else mixed[sl]
+                np.testing.assert_array_equal(out[sl], want)
+        autotune.set_tuning(base)
+        stress = engine.DeviceBatch(workloads.precision_stress(seed=41003, regions=8))
+        rep = autotune.tune_forward(stress, mode=0, reps=1)
+        assert rep["route64"]["flagged_frac"] > 0.25 and "routed_ms" in rep["route64"]
+    finally:
+        autotune.set_tuning(base)
+
+
 def test_uneven_regions_subset():
     b = workloads.uneven_regions(seed=41002, regions=24)
     out, st = engine.forward(b, mode=0, precision="mixed", stats=True)

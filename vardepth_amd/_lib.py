@@ -61,7 +61,8 @@ class VdStats(ctypes.Structure):
 class VdTuning(ctypes.Structure):
     _fields_ = [("fwd_unroll", ctypes.c_int * 17), ("fwd_blocks_per_cu", ctypes.c_int * 17),
                 ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17),
-                ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int), ("class_overlap", ctypes.c_int)]
+                ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int), ("class_overlap", ctypes.c_int),
+                ("fwd_route64", ctypes.c_int * 17)]
     SCALARS = ("share_chunk", "class_overlap")
 
     def as_dict(self):

@@ -15,13 +15,18 @@ passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
   kernel's tail, but where one class holds most of the work they share the
   CUs with it for its whole run, and their time varies more: they are kept
   only when more than 2% faster;
+* forward, precision "mixed": when the FP32 pass flags at least a quarter of
+  the pairs, whether the active classes go straight to the FP64 rescue
+  (``vd_tuning.fwd_route64``; kept when more than 2% faster, and the routed
+  pairs then get their FP64 values);
 * adjoint: register block of 2 steps (two waves per SIMD where that fits) or
   4 steps.
 
 None of them changes the arithmetic of a pair except the suffix
 continuation (which sums the suffix columns through an adjoint and agrees
-with full sweeps to FP32 rounding), so results are otherwise identical for
-every choice; only the time differs.  ``tune_forward`` / ``tune_grad`` apply
+with full sweeps to FP32 rounding) and the FP64 routing (routed pairs get
+their FP64 values), so results are otherwise identical for every choice;
+only the time differs.  ``tune_forward`` / ``tune_grad`` apply
 the per-class best and return a report; ``save`` / ``load`` keep a choice
 (every field of ``vd_tuning``, the stream layout included) in a JSON file
 keyed by workload and mode, and ``load`` gives a field the entry lacks its
@@ -150,10 +155,44 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
                             for _ in range(max(reps, 2)))
     overlap = 1 if overlap_ms[1] < 0.98 * overlap_ms[0] else 0
     lib.vd_class_overlap(overlap)   # kept in vd_tuning.class_overlap with the other choices
-    return {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
-            "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
-            "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
-            "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+    report = {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
+              "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
+              "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
+              "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+    if precision == "mixed":
+        report["route64"] = _tune_route(dev_batch, mode, active, reps)
+    return report
+
+
+def _tune_route(dev_batch, mode, active, reps, min_flagged=0.25):
+    """When the FP32 pass flags at least `min_flagged` of the batch's pairs
+    for the FP64 rescue, time the batch with its active classes sent straight
+    to FP64 (vd_tuning.fwd_route64) and keep that when it is more than 2%
+    faster end to end."""
+    def total():
+        best, st = np.inf, None
+        for _ in range(max(reps, 2)):
+            st = dev_batch.forward(mode=mode, precision="mixed", stats=True)
+            best = min(best, st["device_ms"])
+        return best, st
+    base = get()
+    base_ms, st = total()
+    frac = st["flagged"] / max(st["pairs"], 1)
+    out = {"flagged_frac": round(float(frac), 4), "classes": [], "mixed_ms": round(float(base_ms), 3)}
+    if frac < min_flagged:
+        return out
+    t = _lib.VdTuning.from_buffer_copy(base)
+    classes = [r for r in range(1, 17) if active[r]]
+    for r in classes:
+        t.fwd_route64[r] = 1
+    set_tuning(t)
+    routed_ms, _ = total()
+    out["routed_ms"] = round(float(routed_ms), 3)
+    if routed_ms < 0.98 * base_ms:
+        out["classes"] = classes
+    else:
+        set_tuning(base)
+    return out
 
 
 def tune_grad(dev_batch, mode=0, reps=1, candidates=GRAD_CANDIDATES):
@@ -181,6 +220,7 @@ def defaults() -> _lib.VdTuning:
     for r in range(17):
         t.fwd_unroll[r], t.fwd_blocks_per_cu[r], t.fwd_share[r] = FWD_CANDIDATES[0]
         t.grad_variant[r], t.grad_blocks_per_cu[r] = GRAD_CANDIDATES[0]
+        t.fwd_route64[r] = 0
     t.share_chunk = CHUNK_CANDIDATES[0]
     t.class_overlap = 1
     return t

@@ -334,9 +334,19 @@ void tune_defaults() {
     g.tune.grad_variant[r] = 0;
     g.tune.grad_blocks_per_cu[r] = 0;
     g.tune.fwd_share[r] = 2;
+    g.tune.fwd_route64[r] = 0;
   }
   g.tune.share_chunk = 4;
   g.tune_init = true;
+}
+
+// Plan routing mask (k_prep_reads): bit 0 sends every read to the FP64 path
+// (precision FP64), bit c the reads of class c (vd_tuning.fwd_route64).
+int route_mask(int force64) {
+  if (force64) return 1;
+  int m = 0;
+  for (int r = 1; r <= vd::MAX_R32; ++r) m |= g.tune.fwd_route64[r] ? (1 << r) : 0;
+  return m;
 }
 
 int plan_buffers(const vd_batch* db, int64_t npairs) {
@@ -465,7 +475,8 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   p.split = split;
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
-                             g.tab64.as<vd::QTab<double>>(), anchored, force64, smax, blocks32, blocks32s, unroll32,
+                             g.tab64.as<vd::QTab<double>>(), anchored, route_mask(force64), smax, blocks32, blocks32s,
+                             unroll32,
                              blocks64, g.ws64.as<double>(), g.f64_sorted.as<vd::PairRef>(),
                              g.f64_bins.as<unsigned long long>(), dout, st, stats ? g.ev : nullptr,
                              timing ? g.ev_cls : nullptr, two ? &lane2 : nullptr));
@@ -590,7 +601,8 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   p.class_major = 1;
   p.trie = 0;
   p.trie_chunk = g.tune.share_chunk;
-  HIPCHK(vd::launch_pipeline(batch_dev(db), p, dE, d32, d64, anchored, force64, smax, blocks32, blocks32, unroll32,
+  HIPCHK(vd::launch_pipeline(batch_dev(db), p, dE, d32, d64, anchored, route_mask(force64), smax, blocks32, blocks32,
+                             unroll32,
                              blocks64, reinterpret_cast<double*>(B + o_ws), nullptr, nullptr, dout, st, nullptr,
                              nullptr, nullptr));
   return 0;
@@ -1395,6 +1407,7 @@ int vd_tuning_set(const vd_tuning* t) {
     if (t->grad_variant[r] != 0 && t->grad_variant[r] != 1) return fail("vdhmm: grad_variant must be 0 or 1");
     if (t->fwd_blocks_per_cu[r] < 0 || t->grad_blocks_per_cu[r] < 0) return fail("vdhmm: negative grid size");
     if (t->fwd_share[r] < 0 || t->fwd_share[r] > 3) return fail("vdhmm: fwd_share must be 0, 1, 2 or 3");
+    if (t->fwd_route64[r] != 0 && t->fwd_route64[r] != 1) return fail("vdhmm: fwd_route64 must be 0 or 1");
   }
   if (t->share_chunk < 1 || t->share_chunk > 64) return fail("vdhmm: share_chunk must be in 1..64");
   if (t->class_overlap != 0 && t->class_overlap != 1) return fail("vdhmm: class_overlap must be 0 or 1");

@@ -189,7 +189,8 @@ size_t lds_fwd64(int smax);
 hipError_t occupancy(int anchored, int r, int u, int smax, int* blocks_per_cu);  // r = 0: FP64 kernel
 hipError_t occupancy_trie(int anchored, int r, int sfx, int smax, int* blocks_per_cu);    // prefix-sharing FP32 kernel
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32, int anchored,
-                 int force64, double* out, hipStream_t st);  // p.class_major selects the bucket order
+                 int route64, double* out, hipStream_t st);  // p.class_major selects the bucket order; route64
+                                                              // as k_prep_reads (bit 0: every read to FP64)
 // A second stream for the FP32 class kernels: consecutive classes alternate
 // between the caller's stream and `side`, so that the last units of one
 // persistent kernel overlap the first of the next.  Kernels on `side` use the
@@ -200,7 +201,7 @@ struct SideLane {
   size_t trie_half, sfx_half;
 };
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                           const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
+                           const QTab<double>* tab64, int anchored, int route64, int smax, const int* blocks32,
                            const int* blocks32s,
                            const int* unroll32, int blocks64, double* ws64, PairRef* f64_sorted,
                            unsigned long long* f64_bins, double* out, hipStream_t st, hipEvent_t* ev,

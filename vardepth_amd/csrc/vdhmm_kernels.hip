@@ -39,12 +39,13 @@ namespace vd {
 // ---------------------------------------------------------------------------
 // planning
 
-// Class and normaliser sums of every read: one wave per read, coalesced
+// Class and normaliser sums of every read (route64: bit 0 sends every read to
+// the FP64 path, bit c the reads of class c): one wave per read, coalesced
 // loads of the quality bytes, wave reductions of the per-bin FP64 exponents
 // and of the FP32 ln rho (lane partial sums in a fixed order, then a fixed
 // butterfly, so the FP64 sum of a read does not depend on the batch).
 __global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, const QTab<float>* __restrict__ tab32,
-                             int anchored, int force64, int8_t* __restrict__ cls, int32_t* __restrict__ exps,
+                             int anchored, int route64, int8_t* __restrict__ cls, int32_t* __restrict__ exps,
                              double* __restrict__ ln32) {
   const int lane = threadIdx.x & 63;
   const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -67,7 +68,9 @@ __global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, const Q
     if (lane == 0) {
       const int n = (int)(o1 - o0);
       const int np = n + anchored;
-      const int c = n == 0 ? CLS_EMPTY : ((force64 || np > GROUP * MAX_R32) ? CLS_F64 : (np + GROUP - 1) / GROUP);
+      const int c32 = (np + GROUP - 1) / GROUP;
+      const bool to64 = (route64 & 1) || c32 > MAX_R32 || ((route64 >> c32) & 1);
+      const int c = n == 0 ? CLS_EMPTY : (to64 ? CLS_F64 : c32);
       cls[r] = (int8_t)c;
       exps[r] = e;
       ln32[r] = l;
@@ -1433,13 +1436,13 @@ static void launch_fwd32t_r(int r, int blocks, size_t lds, hipStream_t st, const
 }
 
 void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32, int anchored,
-                 int force64, double* out, hipStream_t st) {
+                 int route64, double* out, hipStream_t st) {
   const int64_t nb = b.nregions * NCLS;
   hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, p.ctr, (int)CTR_COUNT);
   if (b.nreads > 0) {
     int g = (int)((b.nreads + 3) / 4);  // 4 waves of 64 per block, one read per wave
     g = g > 8192 ? 8192 : g;
-    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, tab32, anchored, force64, p.read_cls,
+    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, tab32, anchored, route64, p.read_cls,
                        p.read_exp, p.read_ln32);
   }
   if (b.nregions > 0) {
@@ -1453,16 +1456,16 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const Q
 }
 
 hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
-                           const QTab<double>* tab64, int anchored, int force64, int smax, const int* blocks32,
+                           const QTab<double>* tab64, int anchored, int route64, int smax, const int* blocks32,
                            const int* blocks32s,
                            const int* unroll32, int blocks64, double* ws64, PairRef* f64_sorted,
                            unsigned long long* f64_bins, double* out, hipStream_t st0, hipEvent_t* ev,
                            hipEvent_t* ev_cls, const SideLane* lane2) {
   hipStream_t st = st0;
   if (ev) (void)hipEventRecord(ev[0], st);
-  launch_plan(b, p, qE, tab32, anchored, force64, out, st);
+  launch_plan(b, p, qE, tab32, anchored, route64, out, st);
   if (ev) (void)hipEventRecord(ev[1], st);
-  if (!force64) {
+  if (!(route64 & 1)) {
     // per-class timing keeps one stream, so that each class is timed alone
     const bool two = lane2 && !ev_cls;
     const PlanDev& p0 = p;
