@@ -19,6 +19,9 @@ The JSON line carries:
                 on a time-bounded sample of the same workload, on host threads
 * graph      -- the same step replayed from a captured HIP graph
 * plain      -- the one-pair-per-launch GPU forward on a sample of the workload
+* ref_design -- the reference's own GPU design (one thread per pair, DP rows in
+                global memory, FP32 with its 2^-40 rescale), batched in length
+                buckets, on a sample of whole regions of the same workload
 * fwd_bwd    -- forward + adjoint (gradients wrt gap, ext, 64-bin error table)
                 on the 1024-region HMM-sampled workload (seed 41004, 2.05M pairs);
                 `fused_*` counts the gradient call alone, which also returns
@@ -246,6 +249,35 @@ def plain_baseline(batch, mode: int, pairs: int):
                     done += 1
     return {"gcups": round(cells / ms / 1e6, 4), "pairs": done, "cells": cells, "gpu_ms": round(ms, 3),
             "kind": "one kernel launch per pair, one thread per read row, FP32, inputs resident in HBM"}
+
+
+def refdesign_baseline(batch, mode: int, cells_target: float, engine_gcups: float):
+    """The reference's own GPU design (pairhmm_cuda.cu's one thread per pair,
+    DP rows in global memory, FP32 with its 2^-40 rescale; accelerator.py's
+    length buckets), batched, on a seeded random selection of whole regions
+    of about `cells_target` cells; its NaN pairs are where that design falls
+    back to FP64 (not run here)."""
+    from vardepth_amd import engine
+    rng = np.random.default_rng(12)
+    order = rng.permutation(batch.nregions)
+    rl, hl = batch.read_lengths(), batch.hap_lengths()
+    take, total = [], 0
+    for k in order:
+        take.append(int(k))
+        total += int(rl[batch.region_reads[k]:batch.region_reads[k + 1]].sum()) * \
+            int(hl[batch.region_haps[k]:batch.region_haps[k + 1]].sum())
+        if total >= cells_target:
+            break
+    sub = batch.subset(sorted(take))
+    engine.forward_refdesign(sub.subset([0]), mode=mode)  # warm: module load, buffers
+    out, ms, launches = engine.forward_refdesign(sub, mode=mode)
+    cells = sub.cells()
+    g = cells / ms / 1e6
+    return {"gcups": round(g, 3), "regions": sub.nregions, "pairs": sub.npairs, "cells": cells,
+            "gpu_ms": round(ms, 3), "launches": launches, "nan_frac": round(float(np.isnan(out).mean()), 4),
+            "engine_speedup": round(engine_gcups / g, 2),
+            "kind": "the reference's GPU design (one thread per pair, rows in global memory, FP32 + 2^-40 "
+                    "rescale, 32-base length buckets), batched, inputs resident in HBM"}
 
 
 def pair_lengths(batch):
@@ -576,6 +608,8 @@ def main():
     ap.add_argument("--no-forward", action="store_true",
                     help="skip the forward workload (profiling the fwd+bwd workload alone); prints no JSON line")
     ap.add_argument("--plain-pairs", type=int, default=20000, help="pairs of the plain baseline sample (0 = skip)")
+    ap.add_argument("--ref-cells", type=float, default=5e10,
+                    help="cells of the reference-GPU-design baseline sample (0 = skip)")
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
@@ -680,6 +714,12 @@ def main():
         print("plain one-pair-per-launch: %.3f GCUPS on %d pairs" % (plain["gcups"], plain["pairs"]),
               file=sys.stderr, flush=True)
 
+    ref_design = None
+    if args.ref_cells > 0:
+        ref_design = refdesign_baseline(batch, args.mode, args.ref_cells, gcups)
+        print("reference GPU design: %.2f GCUPS on %d pairs" % (ref_design["gcups"], ref_design["pairs"]),
+              file=sys.stderr, flush=True)
+
     accuracy = None
     if not args.no_accuracy:
         accuracy = accuracy_configs(args.mode)
@@ -735,6 +775,7 @@ def main():
         "cpu_baseline": cpu,
         "graph": graph,
         "plain": plain,
+        "ref_design": ref_design,
         "fwd_bwd": fwd_bwd,
         "accuracy": accuracy,
         "scalar_dropins": scalar,

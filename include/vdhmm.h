@@ -353,6 +353,21 @@ int vd_genotype_guard(const vd_geno_batch *batch, const double *gl, const double
 int vd_forward_plain(const vd_batch *batch, const vd_params *params, int mode, int64_t max_pairs,
                      double *out, double *gpu_ms);
 
+/* The reference's own GPU design, batched, as a second measurement
+ * baseline (no counterpart call in the reference; it restates the strategy
+ * of pairhmm_cuda.cu as dispatched by accelerator.py:80-159 in its "mixed"
+ * precision): one thread per pair sweeping the read rows, the pair's DP row
+ * in global memory with columns interleaved across the pairs of a launch,
+ * FP32 values with an FP64 log scale, a non-anchored row rescaled by 2^64
+ * below 2^-40, an anchored row normalised by its maximum, and NaN for a
+ * terminal below 2^-40 (where the reference falls back to FP64).  Pairs run
+ * in (read, haplotype) length buckets of 32 bases, each in launches whose
+ * workspace stays within budget_mb MiB.  out: every pair of the batch
+ * (ln, NaN, or -inf for an empty read or haplotype).  *gpu_ms: device time
+ * of the launches (inputs staged before); *launches: their count. */
+int vd_forward_refdesign(const vd_batch *batch, const vd_params *params, int mode, int64_t budget_mb,
+                         double *out, double *gpu_ms, int64_t *launches);
+
 /* Longest haplotype (bases) a batch may hold: the forward (gradients = 0)
  * and gradient (gradients = 1) kernels keep each work unit's haplotype codes
  * in one workgroup's LDS (160 KiB on gfx950, about 20,000 bases).  Calls with

@@ -468,3 +468,40 @@ def test_haplotype_length_limit():
     out = engine.forward(workloads.from_regions([([read], [qual], [ok])]), mode=0, precision="mixed")
     expect = oracle.dense(read, qual, ok, 0)
     assert abs(out[0] - expect) <= TOL_MIXED
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reference_design_baseline_vs_oracle(mode):
+    """The batched reference-GPU-design baseline computes the same model: its
+    non-NaN pairs agree with the FP64 oracle to its FP32 accuracy, -inf for
+    empty reads and haplotypes."""
+    if mode == 0:
+        b = workloads.uneven_regions(seed=3, regions=12)
+    else:
+        # anchored (global) alignment: haplotypes about as long as the reads,
+        # otherwise the design's 2^-40 terminal rule sends most pairs to FP64
+        rng = np.random.default_rng(3)
+        regions = []
+        for _ in range(8):
+            L = int(rng.integers(80, 200))
+            haps = ["".join(rng.choice(list("ACGT"), L)) for _ in range(3)]
+            reads, quals = [], []
+            for h in haps * 4:
+                r = list(h)
+                for i in rng.choice(L, 3, replace=False):
+                    r[i] = "ACGT"[(int("ACGT".index(r[i])) + 1) % 4]
+                reads.append("".join(r))
+                quals.append(bytes(rng.integers(20, 41, L).astype(np.uint8)))
+            regions.append((reads, quals, haps))
+        b = workloads.from_regions(regions)
+    edge = workloads.from_regions([(["ACGT", ""], [bytes([30] * 4), b""], ["ACGTA", ""])])
+    for batch in (b, edge):
+        out, ms, launches = engine.forward_refdesign(batch, mode=mode, budget_mb=64)
+        expect, _ = oracle.regions_cpu(batch, mode=mode)
+        assert launches >= 1 and ms > 0
+        np.testing.assert_array_equal(np.isneginf(out), np.isneginf(expect))
+        ok = np.isfinite(out)
+        # anchored: each read against its own haplotype (a third of the pairs)
+        # stays in the design's FP32 window; against unrelated ones it need not
+        assert ok.sum() >= ((0.5 if mode == 0 else 1 / 3) * batch.npairs if batch is b else 1)
+        assert np.abs(out[ok] - expect[ok]).max() <= TOL_MIXED

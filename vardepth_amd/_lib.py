@@ -28,7 +28,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 SYMBOLS = ("pairhmm_pairs", "trie_many", "read_haplotype", "read_haplotype_batch",
            "read_haplotype_batch_mode", "wide_capability", "anchored_pairhmm",
            "vd_core_read_haplotype", "vd_core_read_haplotype_batch", "vd_params_default",
-           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_tuning_get",
+           "vd_forward", "vd_forward_device", "vd_grad", "vd_grad_device", "vd_forward_plain", "vd_forward_refdesign",
+           "vd_tuning_get",
            "vd_tuning_set", "vd_class_timing", "vd_class_ms", "vd_mixture_general", "vd_mixture_two",
            "vd_normalized_rows", "vd_mixture_regions_device", "vd_genotype_device", "vd_genotype_guard", "vd_prefix_sharing",
            "vd_suffix_sharing", "vd_class_overlap", "vd_max_hap_len",
@@ -143,6 +144,9 @@ def lib() -> ctypes.CDLL:
     L.vd_forward_plain.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int, ctypes.c_int64,
                                    _dp, _dp]
     L.vd_forward_plain.restype = ctypes.c_int
+    L.vd_forward_refdesign.argtypes = [ctypes.POINTER(VdBatch), ctypes.POINTER(VdParams), ctypes.c_int,
+                                       ctypes.c_int64, _dp, _dp, _i64p]
+    L.vd_forward_refdesign.restype = ctypes.c_int
     L.vd_tuning_get.argtypes = [ctypes.POINTER(VdTuning)]
     L.vd_tuning_get.restype = ctypes.c_int
     L.vd_tuning_set.argtypes = [ctypes.POINTER(VdTuning)]

@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> Path:
     tmp.mkdir(exist_ok=True)
     for src, lang in ((CSRC / "vdhmm_kernels.hip", "hip"), (CSRC / "vdhmm_grad.hip", "hip"),
                       (CSRC / "vdhmm_plain.hip", "hip"), (CSRC / "vdhmm_genotype.hip", "hip"),
-                      (CSRC / "vdhmm_noalloc.hip", "hip"),
+                      (CSRC / "vdhmm_noalloc.hip", "hip"), (CSRC / "vdhmm_refdesign.hip", "hip"),
                       (CSRC / "vdhmm_engine.cpp", "c++")):
         obj = tmp / (src.stem + ".o")
         cmd = [hc] + HIP_FLAGS + (["-x", "hip"] if lang == "hip" else []) + ["-c", str(src), "-o", str(obj)]

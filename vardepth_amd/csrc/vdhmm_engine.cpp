@@ -909,6 +909,90 @@ int plain_host_locked(const vd_batch* hb, const vd_params* params, int mode, int
   return 0;
 }
 
+// ---- reference-design baseline: one thread per pair, rows in global memory --
+
+int refdesign_host_locked(const vd_batch* hb, const vd_params* params, int mode, int64_t budget_mb, double* out,
+                          double* gpu_ms, int64_t* launches) {
+  int64_t npairs = 0;
+  int max_hap = 0;
+  uint64_t cells = 0;
+  if (int rc = validate_host_batch(hb, &npairs, &max_hap, &cells)) return rc;
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (budget_mb < 8) return fail("vdhmm: the workspace budget must be at least 8 MiB");
+  const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
+  if (int rc = check_params(*params)) return rc;
+  if (int rc = engine_init()) return rc;
+  hipStream_t st = g.own_stream;
+  StreamOrder order(st);
+  if (int rc = stage_host_batch(hb, npairs, st)) return rc;
+  // pairs grouped by (read, haplotype) length in 32-base buckets, as the
+  // reference's dispatcher does for its one-thread-per-pair kernel
+  std::vector<vd::PairRef> list;
+  list.reserve(npairs);
+  std::vector<std::pair<uint64_t, int64_t>> key;  // (bucket, position in list)
+  key.reserve(npairs);
+  for (int64_t k = 0; k < hb->nregions; ++k) {
+    const int64_t r0 = hb->region_reads[k], r1 = hb->region_reads[k + 1];
+    const int64_t h0 = hb->region_haps[k], h1 = hb->region_haps[k + 1];
+    for (int64_t r = r0; r < r1; ++r)
+      for (int64_t h = h0; h < h1; ++h) {
+        const int64_t n = hb->read_off[r + 1] - hb->read_off[r], m = hb->hap_off[h + 1] - hb->hap_off[h];
+        const uint64_t b = ((uint64_t)((n + 31) / 32) << 32) | (uint64_t)((m + 31) / 32);
+        key.emplace_back(b, (int64_t)list.size());
+        list.push_back(vd::PairRef{r, h, hb->out_off[k] + (r - r0) * (h1 - h0) + (h - h0)});
+      }
+  }
+  std::stable_sort(key.begin(), key.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<vd::PairRef> sorted(list.size());
+  for (size_t i = 0; i < key.size(); ++i) sorted[i] = list[key[i].second];
+  struct Owned {  // the baseline's own buffers, freed on every return
+    void* p = nullptr;
+    ~Owned() {
+      if (p) (void)hipFree(p);
+    }
+  } dlist, dws;
+  HIPCHK(hipMalloc(&dlist.p, std::max<size_t>(sorted.size(), 1) * sizeof(vd::PairRef)));
+  if (!sorted.empty())
+    HIPCHK(hipMemcpyAsync(dlist.p, sorted.data(), sorted.size() * sizeof(vd::PairRef), hipMemcpyHostToDevice, st));
+  const size_t budget = (size_t)budget_mb << 20;
+  HIPCHK(hipMalloc(&dws.p, budget));
+  float err[64];
+  for (int q = 0; q < 64; ++q) err[q] = (float)params->err[q];
+  const vd_batch dv = staged_view(hb);
+  const vd::BatchDev b = batch_dev(&dv);
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventRecord(g.ev[0], st));
+  int64_t nl = 0;
+  for (size_t i = 0; i < key.size();) {
+    size_t j = i;
+    while (j < key.size() && key[j].first == key[i].first) ++j;
+    // the bucket's widest haplotype sets the row stride of its launches
+    int cols = 0;
+    for (size_t t = i; t < j; ++t) {
+      const vd::PairRef& pr = sorted[t];
+      cols = std::max<int>(cols, (int)(hb->hap_off[pr.hap + 1] - hb->hap_off[pr.hap]));
+    }
+    const size_t per_pair = (size_t)3 * (cols + 1) * sizeof(float);
+    if (per_pair > budget) return fail("vdhmm: one pair exceeds the baseline's workspace budget");
+    const size_t step = std::max<size_t>(1, std::min<size_t>(budget / per_pair, (size_t)1 << 30));
+    for (size_t s0 = i; s0 < j; s0 += step) {
+      const int cnt = (int)std::min(step, j - s0);
+      HIPCHK(vd::launch_ref_pairs(b, static_cast<vd::PairRef*>(dlist.p) + s0, cnt, cols, err, (float)params->gap,
+                                  (float)params->ext, anchored, static_cast<float*>(dws.p), g.in_out.as<double>(), st));
+      ++nl;
+    }
+    i = j;
+  }
+  HIPCHK(hipEventRecord(g.ev[1], st));
+  HIPCHK(hipEventSynchronize(g.ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, g.ev[0], g.ev[1]));
+  if (gpu_ms) *gpu_ms = ms;
+  if (launches) *launches = nl;
+  if (npairs) HIPCHK(hipMemcpy(out, g.in_out.p, npairs * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // ---- diploid genotype likelihood ------------------------------------------
 
 struct GenoBufs {
@@ -1331,6 +1415,13 @@ int vd_forward_plain(const vd_batch* batch, const vd_params* params, int mode, i
   if (!batch || !params || !out) return fail("vdhmm: null argument");
   EagerLock lk;
   return plain_host_locked(batch, params, mode, max_pairs, out, gpu_ms);
+}
+
+int vd_forward_refdesign(const vd_batch* batch, const vd_params* params, int mode, int64_t budget_mb, double* out,
+                         double* gpu_ms, int64_t* launches) {
+  if (!batch || !params || !out) return fail("vdhmm: null argument");
+  EagerLock lk;
+  return refdesign_host_locked(batch, params, mode, budget_mb, out, gpu_ms, launches);
 }
 
 int vd_mixture_general(int rows, int n, const double* lh, double prior, int triangular, double* out) {

@@ -209,6 +209,8 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
 // FP64 list binning (f64_sorted / f64_bins non-null): capacity of the sorted
 // copy = the list's, bins = 2 * F64_BINS_TOTAL counters
 constexpr int F64_BINS_TOTAL = 16 * 32;
+hipError_t launch_ref_pairs(const BatchDev& b, const PairRef* pairs, int count, int cols, const float* err,
+                            float gap, float ext, int anchored, float* ws, double* out, hipStream_t st);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
 size_t lds_grad32(int smax);

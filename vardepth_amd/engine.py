@@ -102,6 +102,26 @@ def forward_plain(batch: RegionBatch, mode=0, err=None, max_pairs=None):
     return out[:batch.npairs], ms.value
 
 
+def forward_refdesign(batch: RegionBatch, mode=0, err=None, budget_mb=1024):
+    """The reference's own GPU design (one thread per pair, DP rows in global
+    memory, FP32 with its 2^-40 rescale rule), batched in length buckets, as a
+    measurement baseline: returns (ln L with NaN where that design would fall
+    back to FP64, device milliseconds, launches)."""
+    oo = batch.out_off()
+    keep = [batch.read_bytes, batch.qual_bytes, batch.read_off, batch.hap_bytes, batch.hap_off,
+            batch.region_reads, batch.region_haps, oo]
+    keep = [a if a.size else np.zeros(1, a.dtype) for a in keep]
+    vb = _lib.VdBatch(*[_ptr(a) for a in keep], batch.nregions, batch.nreads, batch.nhaps)
+    out = np.full(max(batch.npairs, 1), np.nan)
+    ms = ctypes.c_double(0.0)
+    launches = ctypes.c_int64(0)
+    p = _lib.params(batch.gap, batch.ext, err)
+    code = _lib.lib().vd_forward_refdesign(ctypes.byref(vb), ctypes.byref(p), _mode(mode), int(budget_mb),
+                                           out.ctypes.data_as(_lib._dp), ctypes.byref(ms), ctypes.byref(launches))
+    _lib.check(code, "vd_forward_refdesign")
+    return out[:batch.npairs], ms.value, int(launches.value)
+
+
 GENO_MIN_CERTAINTY = float(np.log(3))   # whole_read.py:37
 GENO_PRIOR = 1e-3                        # whole_read.py:49
 GUARD_MARGIN = 1e-4                      # partial_anchor.py:125: 1e-4 * max(1, molecules)
