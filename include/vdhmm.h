@@ -172,10 +172,13 @@ int vd_forward_device_noalloc(const vd_batch *dev_batch, const vd_params *params
  * the pairs the FP32 pass flags).  grad[0] = d/dgap, grad[1] = d/dext,
  * grad[2 + q] = d/derr[q] (q = 0..63).  weights: one per output slot, NULL =
  * all 1.  out (optional): the ln likelihoods, as vd_forward(mixed) returns.
- * Mode 2 is differentiated as mode 0 (same model).  Reads must be at most 256
- * bases (255 anchored).  The error-table bins enter through the rescaled
- * emissions with each bin's power-of-two normaliser held fixed, so a gradient
- * is exact away from the error values where that normaliser changes.
+ * Mode 2 is differentiated as mode 0 (same model).  Reads of up to 256 rows
+ * (256 bases, 255 anchored) run in the register-resident kernels; longer
+ * reads run in the tiled FP64 adjoint (64-row tiles, each tile's boundary row
+ * kept from the forward, the tiles' adjoints chained bottom to top).  The
+ * error-table bins enter through the rescaled emissions with each bin's
+ * normaliser held fixed; the likelihood does not depend on that choice, so
+ * the derivatives at fixed normalisers are the gradient.
  * Host pointers; returns 0, or 2 with vd_last_error() set. */
 int vd_grad(const vd_batch *batch, const vd_params *params, int mode, const double *weights, double *out,
             double *grad, vd_stats *stats);

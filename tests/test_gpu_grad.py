@@ -156,6 +156,62 @@ def test_grad_fp64_rescue():
     check_grad(g, dg, dx, de)
 
 
+def long_pairs(rng, lengths, extra=(0, 300)):
+    """Reads beyond 256 rows against haplotypes `extra` bases longer, mostly
+    placed in their haplotype with a few substitutions, a few against
+    unrelated haplotypes."""
+    out = []
+    for k, n in enumerate(lengths):
+        m = n + int(rng.integers(*extra))
+        read = rng.choice(list(b"ACGT"), n).astype(np.uint8)
+        if k % 4 != 3 and m > n:
+            hap = rng.choice(list(b"ACGT"), m).astype(np.uint8)
+            o = int(rng.integers(0, m - n + 1))
+            hap[o:o + n] = read
+            for i in rng.choice(n, 4, replace=False):
+                read[i] = b"ACGT"[(b"ACGT".index(int(read[i])) + 1) % 4]
+        else:
+            hap = rng.choice(list(b"ACGT"), m).astype(np.uint8)
+        out.append((read.tobytes(), rng.integers(10, 41, n).astype(np.uint8).tobytes(), hap.tobytes()))
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_grad_long_reads_tiled(mode):
+    """Reads of 257-700 bases (up to 11 tiles of 64 rows, unit mates with
+    different tile counts) run in the tiled FP64 adjoint, together with short
+    reads in the FP32 kernels: ln L and gradients against the float64
+    autograd restatement."""
+    rng = np.random.default_rng(40 + mode)
+    lengths = [257, 300, 700, 380, 450, 640, 511, 600, 256 - mode, 320]
+    # anchored: haplotypes only slightly longer than the reads, so that the
+    # terminal stays above the reference's 1e-300 clamp the restatement omits
+    pairs = long_pairs(rng, lengths, (0, 300) if mode == 0 else (1, 24))
+    pairs += rand_pairs(rng, 6)
+    err = perturbed_err(3)
+    gap, ext = 0.004, 0.15
+    w = rng.uniform(0.5, 2.0, len(pairs))
+    long_ = np.array([len(p_[0]) + mode > 256 for p_ in pairs])
+    only, wl = [p_ for p_, k in zip(pairs, long_) if k], w[long_]
+    short, ws = [p_ for p_, k in zip(pairs, long_) if not k], w[~long_]
+    vl, dgl, dxl, del_ = torch_ref.value_and_grad(only, mode, gap, ext, err, wl)
+    vs, dgs, dxs, des = torch_ref.value_and_grad(short, mode, gap, ext, err, ws)
+    # the long pairs alone: all FP64, ln L and the gradient to FP64 rounding
+    outl, gl, st = engine.grad(batch_of(only, gap, ext), mode=mode, err=err, weights=wl, stats=True)
+    np.testing.assert_allclose(outl, vl, rtol=0, atol=1e-9)
+    check_grad(gl, dgl, dxl, del_, rtol=1e-7)
+    assert st["fp64_bad"] == 0
+    # with the short reads (FP32 kernels) in one call: the sum of both
+    out, g = engine.grad(batch_of(pairs, gap, ext), mode=mode, err=err, weights=w)
+    np.testing.assert_allclose(out[long_], vl, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(out[~long_], vs, rtol=0, atol=TOL_MIXED)
+    check_grad(g, dgl + dgs, dxl + dxs, del_ + des)
+    # a gradient is a sum over pairs: each long pair alone
+    parts = [engine.grad(batch_of([p_], gap, ext), mode=mode, err=err, weights=wl[i:i + 1])[1]
+             for i, p_ in enumerate(only)]
+    np.testing.assert_allclose(gl, np.sum(parts, axis=0), rtol=1e-9, atol=1e-9 * np.max(np.abs(gl)))
+
+
 def test_grad_empty_inputs():
     rng = np.random.default_rng(3)
     pairs = rand_pairs(rng, 6)

@@ -145,7 +145,7 @@ struct Engine {
   // parameter tables
   DevBuf qE, tab32, tab64, gtab;
   // gradient scratch: checkpoints, per-block partial gradients, staged weights / gradient
-  DevBuf ck32, ck64, gpart, bins64, in_w, in_grad;
+  DevBuf ck32, ck64, gpart, bins64, in_w, in_grad, ws64t;
   vd_params params_on_device;
   bool have_params = false;
   int occ32[2][17][2] = {}, occ64[2] = {0, 0};  // [anchored][class][unroll 2 / 4]
@@ -723,8 +723,6 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   if (db->nregions < 0 || db->nreads < 0 || db->nhaps < 0 || npairs < 0 || max_hap_len < 0 || max_read_len < 0)
     return fail("vdhmm: negative batch size");
   const int anchored = mode == VD_MODE_ANCHORED ? 1 : 0;
-  if (max_read_len + anchored > vd::GROUP * vd::MAX_R32)
-    return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
   if (int rc = check_params(*params)) return rc;
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = engine_init()) return rc;
@@ -759,17 +757,32 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   }
   max_blocks = std::max(max_blocks, max64);
   const int64_t bin_cap = std::max<int64_t>(npairs, 1);
+  // reads beyond 256 rows: the tiled FP64 adjoint, 64-row tiles, its
+  // persistent grid bounded by the FP64 kernels' (checkpoint areas) and by
+  // a 2 GiB tile workspace
+  vd::GradTiled tiled{0, 0, nullptr, 0};
+  if (max_read_len + anchored > vd::GROUP * vd::MAX_R32) {
+    tiled.ntmax = (max_read_len + anchored + vd::GROUP * vd::R64 - 1) / (vd::GROUP * vd::R64);
+    tiled.ws_stride = vd::grad_tile_ws(smax, tiled.ntmax);
+    int occ = 1;
+    HIPCHK(vd::grad_tiled_occupancy(anchored, smax, &occ));
+    const size_t cap_ws = ((size_t)2 << 30) / (tiled.ws_stride * sizeof(double));
+    tiled.blocks = (int)std::max<size_t>(
+        1, std::min<size_t>({(size_t)g.cus * std::max(1, std::min(occ, 16)), (size_t)max64, cap_ws}));
+    HIPCHK(g.ws64t.ensure((size_t)tiled.blocks * tiled.ws_stride * sizeof(double)));
+    tiled.ws = g.ws64t.as<double>();
+  }
   HIPCHK(g.ck32.ensure((size_t)max_blocks * vd::grad_ck_stride(smax, 0) * sizeof(float)));
   HIPCHK(g.ck64.ensure((size_t)max64 * vd::grad_ck_stride(smax, 1) * sizeof(double)));
   HIPCHK(g.gpart.ensure((size_t)max_blocks * vd::NGRAD * sizeof(double)));
-  HIPCHK(g.bins64.ensure((size_t)8 * bin_cap * sizeof(vd::PairRef)));
+  HIPCHK(g.bins64.ensure((size_t)9 * bin_cap * sizeof(vd::PairRef)));  // 8 row bins + the tiled bin
   const vd::BatchDev b = batch_dev(db);
   const vd::PlanDev p = plan_dev(npairs, 1);
   HIPCHK(vd::launch_grad(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(), g.tab64.as<vd::QTab<double>>(),
                          g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, variant32, blocks64, max_blocks,
                          g.ck32.as<float>(), g.ck64.as<double>(), g.bins64.as<vd::PairRef>(), bin_cap, dweights,
                          dout, g.gpart.as<double>(), dgrad, st, stats ? g.ev : nullptr,
-                         stats && g.class_timing ? g.ev_cls : nullptr));
+                         stats && g.class_timing ? g.ev_cls : nullptr, &tiled));
   if (stats && g.class_timing) {
     HIPCHK(hipEventSynchronize(g.ev[3]));
     for (int r = 1; r <= vd::MAX_R32; ++r) {
@@ -814,8 +827,6 @@ int grad_host_locked(const vd_batch* hb, const vd_params* params, int mode, cons
   for (int64_t i = 0; i < hb->nreads; ++i)
     max_read = std::max<int>(max_read, (int)std::min<int64_t>(hb->read_off[i + 1] - hb->read_off[i], 1 << 30));
   if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
-  if (max_read + (mode == VD_MODE_ANCHORED ? 1 : 0) > vd::GROUP * vd::MAX_R32)
-    return fail("vdhmm: gradients support reads of at most 256 bases (255 in anchored mode)");
   if (int rc = check_params(*params)) return rc;
   if (!(params->ext > 0)) return fail("vdhmm: gradients need a gap continuation in (0, 1)");
   if (int rc = check_hap_len(max_hap, 1)) return rc;

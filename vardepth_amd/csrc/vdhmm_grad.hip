@@ -463,18 +463,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   grad32_body<ANCH, R, GK32>(b, p, gtab, gt, smax, ckall, ck_stride, weights, out, gpart);
 }
 
-// Bins the FP64 gradient list (the pairs the FP32 pass flagged) by rows per
-// lane, rounded up to an even count, so that each FP64 kernel runs with the
-// register footprint of its bin.
+// Bins the FP64 gradient list (the pairs the FP32 pass flagged, and the reads
+// beyond 256 rows) by rows per lane, rounded up to an even count, so that each
+// FP64 kernel runs with the register footprint of its bin; reads beyond 16
+// rows per lane go to bin NBIN64, the tiled adjoint's.
 template <bool ANCH>
 __global__ __launch_bounds__(256) void k_bin64(BatchDev b, PlanDev p, PairRef* __restrict__ bins, int64_t bin_cap) {
-  __shared__ unsigned int cnt[NBIN64];
-  __shared__ unsigned long long base[NBIN64];
+  __shared__ unsigned int cnt[NBIN64 + 1];
+  __shared__ unsigned long long base[NBIN64 + 1];
   unsigned long long count = p.ctr[CTR_F64];
   if ((int64_t)count > p.f64_cap) count = (unsigned long long)p.f64_cap;
   for (unsigned long long i0 = (unsigned long long)blockIdx.x * blockDim.x; i0 < count;
        i0 += (unsigned long long)gridDim.x * blockDim.x) {
-    if (threadIdx.x < NBIN64) cnt[threadIdx.x] = 0;
+    if (threadIdx.x <= NBIN64) cnt[threadIdx.x] = 0;
     __syncthreads();
     const unsigned long long i = i0 + threadIdx.x;
     int bin = -1;
@@ -484,14 +485,14 @@ __global__ __launch_bounds__(256) void k_bin64(BatchDev b, PlanDev p, PairRef* _
       pr = p.f64_list[i];
       const int np = (int)(b.read_off[pr.read + 1] - b.read_off[pr.read]) + (ANCH ? 1 : 0);
       const int rows = (np + GROUP - 1) / GROUP;
-      if (rows <= MAX_R32) {  // the host rejects longer reads before launching
-        bin = (rows + 1) / 2 - 1;
-        slot = atomicAdd(&cnt[bin], 1u);
-      }
+      bin = rows <= MAX_R32 ? (rows + 1) / 2 - 1 : NBIN64;
+      slot = atomicAdd(&cnt[bin], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < NBIN64)
-      base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&p.ctr[CTR_BIN64 + threadIdx.x], (unsigned long long)cnt[threadIdx.x]) : 0ull;
+    if (threadIdx.x <= NBIN64) {
+      const int c = threadIdx.x < NBIN64 ? CTR_BIN64 + (int)threadIdx.x : CTR_BIN64T;
+      base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&p.ctr[c], (unsigned long long)cnt[threadIdx.x]) : 0ull;
+    }
     __syncthreads();
     if (bin >= 0) {
       const unsigned long long k = base[bin] + slot;
@@ -545,6 +546,431 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
     gp.m_min = mn;
     if (!gp.valid) gp.m = mx;
     unit_grad<double, R, ANCH, GK64>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, true);
+  }
+  grad_flush(acc[0], acc[1], lds_de, gpart);
+}
+
+// ---------------------------------------------------------------------------
+// Tiled FP64 adjoint for reads beyond 256 rows (no counterpart in the
+// reference, like every adjoint here).  A unit is 4 listed pairs of the long
+// bin, 16 lanes per pair, R64 = 4 rows per lane: the FP64 forward's 64-row
+// tiles.
+//
+// Phase A is the tiled FP64 forward (tile64, as k_fwd64 runs it), keeping
+// every tile's last row (the P and Q streams lane 15 hands down) and the
+// power-of-two factors its successor's injection is scaled by, and the
+// pair's terminal, ln L and seed.
+//
+// Phase B takes the tiles last to first.  A tile runs unit_grad's three
+// sweeps on its 64 rows: forward from its injection (the previous tile's
+// stored row times its factors, or the model's start for the first tile)
+// with checkpoints, then backward block by block.  Its backward seeds at the
+// bottom are the terminal seeds for the last tile and otherwise the adjoints
+// of the tile below's injection, per column; its own injection's adjoints
+// (lane 0's outgoing Pb and EM, per column) are written out, times the
+// injection's factors, for the tile above.  Every gradient sum is a product
+// of a forward value and an adjoint of the same tile, so the tile scales
+// cancel in it; the sums of the tiles add up.
+
+constexpr int GKT = 2;  // register block (steps) of the tiled adjoint
+constexpr double C64T = 0x1p500;  // tile64's start scale
+
+// Tile workspace (doubles) of one block, for tiles of at most ntmax per pair:
+//   stored forward rows: tile k < ntmax - 1, stream s (P, Q), group g at ((k * 2 + s) * 4 + g) * smax
+//   adjoint rows (two buffers): A + ((buf * 2 + s) * 4 + g) * smax, A = (ntmax - 1) * 8 * smax
+//   injection factors of tile k: F + (k * 4 + g) * 2 + {0, 1}, F = A + 16 * smax
+size_t grad_tile_ws(int smax, int ntmax) {
+  return ((size_t)(ntmax > 1 ? ntmax - 1 : 0) * 8 + 16) * (size_t)smax + (size_t)ntmax * 8;
+}
+
+template <bool ANCH, bool FIRST, bool LAST>
+__device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t* cl, int tend, const double (&gd)[R64],
+                                            const double (&bd)[R64], const double (&mu)[R64],
+                                            const double (&be)[R64], const uint32_t (&rc)[R64],
+                                            const bool (&pass)[R64], double gam, double gam_last, double xx,
+                                            const double* __restrict__ inP, const double* __restrict__ inQ,
+                                            double sc1, double sc2, const double* __restrict__ bP,
+                                            const double* __restrict__ bQ, double* __restrict__ tP,
+                                            double* __restrict__ tQ, double tseed, double* __restrict__ ck,
+                                            double (&Er)[R64], double& U, double& V, double& UL, double& VL,
+                                            double& Tg, double& Tl, double& Tx) {
+  constexpr int R = R64, CKW = 2 * R + 2, CK_STEPS = GKT * NSUB;
+  const int lane = threadIdx.x & 63, l = lane & 15;
+  // this tile's injection at step t (lane 0 takes it through DPP's old operand)
+  auto inj = [&](int t, double& pinj, double& qinj) {
+    if (FIRST) {
+      injections<double, ANCH>(t, gp.m, C64T, pinj, qinj);
+    } else {
+      pinj = qinj = 0.0;
+      if (t - 1 <= gp.m_max) {  // column t - 1, slot t + 16: the slots tile64 loads
+        pinj = inP[t + 16] * sc1 * sc2;
+        qinj = inQ[t + 16] * sc1 * sc2;
+      }
+    }
+  };
+  // ---- sweep 1: forward with checkpoints
+  double M[R], D[R], Q[R], qu[R], dd[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.0;
+  double Plast = 0.0, Xprev = 0.0;
+  for (int t = 0; t <= tend; ++t) {
+    if (t % CK_STEPS == 0) {
+      double* c = ck + (size_t)(t / CK_STEPS) * CKW * 64 + lane;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        c[(size_t)r * 64] = fmaT(xx, D[r], M[r]);
+        c[(size_t)(R + r) * 64] = Q[r];
+      }
+      c[(size_t)(2 * R) * 64] = Plast;
+      c[(size_t)(2 * R + 1) * 64] = Xprev;
+    }
+    double pinj, qinj;
+    inj(t, pinj, qinj);
+    const double Pin = dpp_shr1(pinj, Plast);
+    const double Xn = dpp_shr1(qinj, Q[R - 1]);
+    Plast = gstep<double, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
+    Xprev = Xn;
+  }
+  // ---- sweep 2: backward, block by block
+  // lane 15's seeds from below: the terminal's for the last tile, otherwise
+  // the tile below's injection adjoints (already times its factors):
+  // Pb of column t - 16 at step t (slot t + 1), and Qb of column t - 17
+  // handed in at step t for step t - 1 (slot t)
+  auto seed_p = [&](int t) -> double { return LAST ? (ANCH ? 0.0 : tseed) : bP[t + 1]; };
+  auto seed_q = [&](int t) -> double {
+    return LAST ? ((ANCH && t - 17 == gp.m) ? tseed : 0.0) : bQ[t];
+  };
+  double EMp[R], Dbr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) EMp[r] = Dbr[r] = 0.0;
+  double Ilast = 0.0;
+  double Eprev = l == GROUP - 1 ? (LAST ? ((ANCH && tend - 16 == gp.m) ? tseed : 0.0) : bQ[tend + 1]) : 0.0;
+  const int nblk = tend / CK_STEPS + 1;
+  double cur[CKW];
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    {
+      const double* c = ck + (size_t)blk * CKW * 64 + lane;
+#pragma unroll
+      for (int w = 0; w < CKW; ++w) cur[w] = c[(size_t)w * 64];
+    }
+    const int tb = blk * CK_STEPS;
+    for (int sub = NSUB - 1; sub >= 0; --sub) {
+      const int t0 = tb + sub * GKT;
+      if (t0 > tend) continue;
+      const int cnt = min(GKT, tend + 1 - t0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        M[r] = cur[r];
+        D[r] = 0.0;
+        Q[r] = cur[R + r];
+      }
+      Plast = cur[2 * R];
+      Xprev = cur[2 * R + 1];
+      for (int t = tb; t < t0; ++t) {
+        double pinj, qinj;
+        inj(t, pinj, qinj);
+        const double Pin = dpp_shr1(pinj, Plast);
+        const double Xn = dpp_shr1(qinj, Q[R - 1]);
+        Plast = gstep<double, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
+        Xprev = Xn;
+      }
+      double sQ[GKT][R], sD[GKT][R], sPin[GKT];
+#pragma unroll
+      for (int s = 0; s < GKT; ++s) {
+        if (s < cnt) {
+          const int t = t0 + s;
+          double pinj, qinj;
+          inj(t, pinj, qinj);
+          const double Pin = dpp_shr1(pinj, Plast);
+          const double Xn = dpp_shr1(qinj, Q[R - 1]);
+          Plast = gstep<double, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], sQ[s], sD[s]);
+          Xprev = Xn;
+          sPin[s] = Pin;
+        }
+      }
+#pragma unroll
+      for (int s = GKT - 1; s >= 0; --s) {
+        if (s < cnt) {
+          const int t = t0 + s;
+          const uint32_t code = cl[t];
+          const bool post = code == SENT_POST;
+          const double smis = post ? 0.0 : 1.0 / 3.0;
+          const double Pbin = dpp_shl1(seed_p(t), Ilast);
+          const double Qbin = Eprev;
+          const double En = dpp_shl1(seed_q(t), EMp[0]);
+          double Qb[R], e[R], sw[R], Iv[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
+          double Ih = sPin[s];
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const bool mt = rc[r] == code;
+            e[r] = post ? 0.0 : (mt ? gd[r] : bd[r]);
+            sw[r] = mt ? -1.0 : smis;
+            Iv[r] = pass[r] ? 0.0 : Ih;
+            Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
+          }
+          double Pb = Pbin;
+#pragma unroll
+          for (int r = R - 1; r >= 0; --r) {
+            const double gr = r == R - 1 ? gam_last : gam;
+            const double Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
+            const double Mbar = Pb + Qb[r] + Dbr[r];
+            const double EM = e[r] * Mbar;
+            Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
+            const double u = mu[r] * Pb, v = be[r] * Qb[r];
+            if (r == R - 1) {
+              UL = fmaT(u, Iv[r], UL);
+              VL = fmaT(v, Iv[r], VL);
+              Tl = fmaT(Qb[r], sD[s][r], Tl);
+            } else {
+              U = fmaT(u, Iv[r], U);
+              V = fmaT(v, Iv[r], V);
+              Tg = fmaT(Qb[r], sD[s][r], Tg);
+            }
+            Tx = fmaT(Dbr[r], sD[s][r], Tx);
+            Dbr[r] = Dbar;
+            EMp[r] = EM;
+            Pb = u + v;
+          }
+          Ilast = Pb;
+          Eprev = En;
+          // lane 0: adjoints of this step's injection, Pin (column t - 1) and
+          // the diagonal source Qin (column t - 2), times the injection's
+          // factors for the tile above
+          if (!FIRST && l == 0) {
+            tP[t + 16] = Pb * sc1 * sc2;
+            tQ[t + 15] = EMp[0] * sc1 * sc2;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <bool ANCH>
+__global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
+                                                const GradTab* __restrict__ gt, int smax, int ntmax,
+                                                double* __restrict__ ckall, size_t ck_stride,
+                                                double* __restrict__ wsall, size_t ws_stride,
+                                                const PairRef* __restrict__ bins, int64_t bin_cap,
+                                                const double* __restrict__ weights, double* __restrict__ out,
+                                                double* __restrict__ gpart) {
+  constexpr int R = R64;
+  extern __shared__ __align__(16) unsigned char lds[];
+  grad_lds_init<double>(gtab, qtab_lds_bytes<double>(), lds);
+  double* lds_de = reinterpret_cast<double*>(lds);
+  const QTab<double>* tab = reinterpret_cast<const QTab<double>*>(lds + 64 * sizeof(double));
+  uint16_t* codes =
+      reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
+  double* ck = ckall + (size_t)blockIdx.x * ck_stride;
+  double* ws = wsall + (size_t)blockIdx.x * ws_stride;
+  const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+  double* const adjb = ws + (size_t)(ntmax > 1 ? ntmax - 1 : 0) * 8 * smax;
+  double* const fac = adjb + (size_t)16 * smax;
+  double acc[2] = {0.0, 0.0};
+  unsigned long long count = p.ctr[CTR_BIN64T];
+  if ((int64_t)count > bin_cap) count = (unsigned long long)bin_cap;
+  const PairRef* list = bins + (size_t)NBIN64 * bin_cap;
+  const unsigned long long units = (count + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
+  for (;;) {
+    unsigned long long u = 0;
+    if (lane == 0) u = atomicAdd(&p.ctr[CTR_UNIT64], 1ull);
+    u = first_lane(u);
+    if (u >= units) break;
+    const unsigned long long idx = u * PAIRS_PER_UNIT + g;
+    GroupPair gp;
+    gp.valid = idx < count;
+    const PairRef pr = list[gp.valid ? idx : u * PAIRS_PER_UNIT];
+    gp.read = pr.read;
+    gp.hap = pr.hap;
+    gp.out = pr.out;
+    gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+    gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+    const int np0 = gp.n + (ANCH ? 1 : 0);
+    if (gp.valid && (gp.m == 0 || np0 > ntmax * GROUP * R)) gp.valid = 0;
+    const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+    const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+    const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
+    if (mx == 0) continue;
+    gp.m_max = mx;
+    gp.m_min = mn;
+    if (!gp.valid) gp.m = mx;
+    const int tend = gp.m_max + 16;
+    const int tmain = gp.m_min + 2;
+    fill_codes(b, gp, codes, smax, tend + 18);
+    const int np = gp.n + (ANCH ? 1 : 0);
+    const int ntile_g = gp.valid ? (np + GROUP * R - 1) / (GROUP * R) : 0;
+    const int ntiles = max(max(lane16(ntile_g, 0), lane16(ntile_g, 1)), max(lane16(ntile_g, 2), lane16(ntile_g, 3)));
+    const int pad = ntiles * GROUP * R - np;
+    const int64_t rb = b.read_off[gp.read];
+    const double gam = tab->gamma, xx = tab->ext;
+    const uint16_t* cl = codes + g * smax + 16 - l;
+    auto rowP = [&](int k) { return ws + ((size_t)(k * 2 + 0) * PAIRS_PER_UNIT + g) * smax; };
+    auto rowQ = [&](int k) { return ws + ((size_t)(k * 2 + 1) * PAIRS_PER_UNIT + g) * smax; };
+    auto adjP = [&](int k) { return adjb + ((size_t)((k & 1) * 2 + 0) * PAIRS_PER_UNIT + g) * smax; };
+    auto adjQ = [&](int k) { return adjb + ((size_t)((k & 1) * 2 + 1) * PAIRS_PER_UNIT + g) * smax; };
+    // row constants of tile `tile` (as unit64)
+    double gd[R], bd[R], mu[R], be[R];
+    uint32_t rc[R];
+    bool pass[R];
+    int qrow[R];
+    auto rows = [&](int tile) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int rel = tile * GROUP * R + l * R + r - pad;
+        double vg = 0.0, vb = 0.0, vm = 0.0, ve = 0.0;
+        uint32_t vc = NOMATCH;
+        int qq = -1;
+        if (gp.valid) {
+          if (rel < 0) {
+            vm = ve = 1.0;
+          } else if (ANCH && rel == 0) {
+            vg = vb = 1.0;
+          } else {
+            const int kk = rel - (ANCH ? 1 : 0);
+            int q = b.qual_bytes[rb + kk];
+            q = q < 63 ? q : 63;
+            vc = b.read_bytes[rb + kk];
+            vg = tab->good[q];
+            vb = tab->bad[q];
+            vm = tab->mu[q];
+            ve = tab->beta[q];
+            if (kk == gp.n - 1) vm = ve = tab->cfin[q];
+            qq = q;
+          }
+        }
+        gd[r] = vg; bd[r] = vb; mu[r] = vm; be[r] = ve; rc[r] = vc;
+        pass[r] = rel < 0;
+        qrow[r] = qq;
+      }
+    };
+    // ---- phase A: the tiled forward, keeping every tile's last row and factors
+    int shift_total = 0;
+    double sc1 = 1.0, sc2 = 1.0, tacc = 0.0, rmax = 0.0, latch = 0.0;
+    if (l == 0) {
+      fac[(0 * PAIRS_PER_UNIT + g) * 2 + 0] = 1.0;
+      fac[(0 * PAIRS_PER_UNIT + g) * 2 + 1] = 1.0;
+    }
+    __syncthreads();
+    for (int tile = 0; tile < ntiles; ++tile) {
+      const bool first = tile == 0, last = tile == ntiles - 1;
+      const double gam_last = (l == GROUP - 1 && last) ? (ANCH ? tab->gap : 0.0) : gam;
+      rows(tile);
+      const double* inP = first ? nullptr : rowP(tile - 1);
+      const double* inQ = first ? nullptr : rowQ(tile - 1);
+      double* oP = last ? nullptr : rowP(tile);
+      double* oQ = last ? nullptr : rowQ(tile);
+      double bmax = 0.0;
+#define VD_TILE(F, L)                                                                                             \
+  tile64<ANCH, F, L>(gp, cl, tmain, tend, gd, bd, mu, be, rc, gam, gam_last, xx, sc1, sc2, inP, inQ, oP, oQ, tacc, \
+                     rmax, latch, bmax)
+      if (first && last) VD_TILE(true, true);
+      else if (first) VD_TILE(true, false);
+      else if (last) VD_TILE(false, true);
+      else VD_TILE(false, false);
+#undef VD_TILE
+      if (!last) {
+        int e = 0;
+        if (bmax > 0.0) frexp(bmax, &e);
+        const bool real_rows = tile >= ntiles - ntile_g;
+        int s = (bmax > 0.0 && real_rows) ? CEXP64 - e : 0;
+        s = s > 2040 ? 2040 : s;
+        const int src = g * GROUP + GROUP - 1;
+        const int s1 = s / 2, s2 = s - s1;
+        sc1 = __shfl(ldexp(1.0, s1), src);
+        sc2 = __shfl(ldexp(1.0, s2), src);
+        shift_total -= __shfl(s, src);
+        if (l == 0) {
+          fac[((tile + 1) * PAIRS_PER_UNIT + g) * 2 + 0] = sc1;
+          fac[((tile + 1) * PAIRS_PER_UNIT + g) * 2 + 1] = sc2;
+        }
+        __threadfence();
+        __syncthreads();
+      }
+    }
+    // terminal, ln L, flag and seed (lane 15 of each group), then the seed to the group
+    bool flag = false;
+    double seed = 0.0, wpair = 0.0;
+    if (l == GROUP - 1 && gp.valid) {
+      const double term = ANCH ? latch : tacc;
+      const double lnk = (double)(p.read_exp[gp.read] - CEXP64 + shift_total) * LN2;
+      double v;
+      if (ANCH) {
+        const double lc = log(rmax) + LN_1EM300;
+        const double ll = log(latch);
+        v = (ll > lc ? ll : lc) + lnk;
+      } else {
+        v = log(tacc) + lnk - log((double)(gp.m + 1));
+      }
+      flag = !(term > 0.0) || !(term < __builtin_inf());
+      if (flag) atomicAdd(&p.ctr[CTR_F64BAD], 1ull);
+      out[gp.out] = v;
+      wpair = weights ? weights[gp.out] : 1.0;
+      seed = flag ? 0.0 : wpair / term;
+    }
+    seed = __shfl(seed, g * GROUP + GROUP - 1);
+    flag = __shfl((int)flag, g * GROUP + GROUP - 1) != 0;
+    // ---- phase B: the tiles last to first
+    for (int k = lane; k < 4 * smax; k += 64) {  // both adjoint buffers of this group start at zero
+      const int s_ = k / smax, slot = k % smax;
+      adjb[((size_t)s_ * PAIRS_PER_UNIT + g) * smax + slot] = 0.0;
+    }
+    __threadfence();
+    __syncthreads();
+    double U = 0.0, V = 0.0, UL = 0.0, VL = 0.0, Tg = 0.0, Tl = 0.0, Tx = 0.0;
+    double Ufin = 0.0, Vfin = 0.0, Tfin = 0.0;  // the read's last row: lane 15, register R-1 of the last tile
+    for (int tile = ntiles - 1; tile >= 0; --tile) {
+      const bool first = tile == 0, last = tile == ntiles - 1;
+      const double gam_last = (l == GROUP - 1 && last) ? (ANCH ? tab->gap : 0.0) : gam;
+      rows(tile);
+      const double* inP = first ? nullptr : rowP(tile - 1);
+      const double* inQ = first ? nullptr : rowQ(tile - 1);
+      const double f1 = fac[(tile * PAIRS_PER_UNIT + g) * 2 + 0], f2 = fac[(tile * PAIRS_PER_UNIT + g) * 2 + 1];
+      const double* bPp = last ? nullptr : adjP(tile + 1);
+      const double* bQp = last ? nullptr : adjQ(tile + 1);
+      double* tPp = first ? nullptr : adjP(tile);
+      double* tQp = first ? nullptr : adjQ(tile);
+      double Er[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) Er[r] = 0.0;
+      const double ts = flag ? 0.0 : seed;
+#define VD_TG(F, L)                                                                                                 \
+  tile_grad64<ANCH, F, L>(gp, cl, tend, gd, bd, mu, be, rc, pass, gam, gam_last, xx, inP, inQ, f1, f2, bPp, bQp, tPp, \
+                          tQp, ts, ck, Er, U, V, UL, VL, Tg, Tl, Tx)
+      if (first && last) VD_TG(true, true);
+      else if (first) VD_TG(true, false);
+      else if (last) VD_TG(false, true);
+      else VD_TG(false, false);
+#undef VD_TG
+      // in every tile but the last, lane 15's last register row is an interior row
+      if (last && l == GROUP - 1) {
+        Ufin = UL;
+        Vfin = VL;
+        Tfin = Tl;
+      } else {
+        U += UL;
+        V += VL;
+        Tg += Tl;
+      }
+      UL = VL = Tl = 0.0;
+      if (gp.valid && !flag) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (qrow[r] >= 0) atomicAdd(&lds_de[qrow[r]], Er[r] * gt->dgood_de[qrow[r]] * -1.0);
+      }
+      __threadfence();
+      __syncthreads();
+    }
+    if (gp.valid && !flag) {
+      const double gg = (double)tab->gap, x = (double)tab->ext, a = 1.0 - 2.0 * gg;
+      const bool last_lane = l == GROUP - 1;
+      const double ww = Ufin + Vfin;  // zero off lane 15
+      const double mocc = last_lane ? (double)gp.n * wpair : 0.0;
+      acc[0] += -2.0 / a * (mocc - (U + V + ww)) + V / (gg * a) + ww / gg + Tg * gt->dgam_dg + (ANCH ? Tfin : 0.0);
+      acc[1] += U / x - V / (1.0 - x) + Tg * gt->dgam_dx + Tx;
+    }
+    __syncthreads();  // codes reused by the next unit
   }
   grad_flush(acc[0], acc[1], lds_de, gpart);
 }
@@ -656,11 +1082,17 @@ static void launch_grad64_bin(int bin, dim3 grid, size_t lds, hipStream_t st, co
   }
 }
 
+hipError_t grad_tiled_occupancy(int anchored, int smax, int* blocks_per_cu) {
+  return anchored ? hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad64t<true>, 64, lds_grad64(smax))
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_grad64t<false>, 64, lds_grad64(smax));
+}
+
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                        const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
                        const int* blocks32, const int* variant32, const int* blocks64, int max_blocks, float* ck32,
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
-                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls) {
+                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls,
+                       const GradTiled* tiled) {
   if (ev) (void)hipEventRecord(ev[0], st);
   launch_plan(b, p, qE, tab32, anchored, 0, out, st);
   hipLaunchKernelGGL(k_zero_f64, dim3(64), dim3(256), 0, st, gpart, (int64_t)max_blocks * NGRAD);
@@ -689,6 +1121,17 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
     else
       launch_grad64_bin<false>(bin, dim3(blocks64[bin]), lds_grad64(smax), st, b, p, tab64, gt, smax, ck64, s64,
                                bins, bin_cap, weights, out, gpart);
+  }
+  if (tiled && tiled->ntmax > 0) {
+    hipLaunchKernelGGL(k_reset_unit64, dim3(1), dim3(64), 0, st, p.ctr);
+    if (anchored)
+      hipLaunchKernelGGL(k_grad64t<true>, dim3(tiled->blocks), dim3(64), lds_grad64(smax), st, b, p, tab64, gt, smax,
+                         tiled->ntmax, ck64, s64, tiled->ws, tiled->ws_stride, (const PairRef*)bins, bin_cap, weights,
+                         out, gpart);
+    else
+      hipLaunchKernelGGL(k_grad64t<false>, dim3(tiled->blocks), dim3(64), lds_grad64(smax), st, b, p, tab64, gt,
+                         smax, tiled->ntmax, ck64, s64, tiled->ws, tiled->ws_stride, (const PairRef*)bins, bin_cap,
+                         weights, out, gpart);
   }
   hipLaunchKernelGGL(k_grad_reduce, dim3(NGRAD), dim3(256), 0, st, (const double*)gpart, max_blocks, grad);
   if (ev) (void)hipEventRecord(ev[3], st);

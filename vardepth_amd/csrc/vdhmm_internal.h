@@ -147,7 +147,8 @@ enum {
   CTR_CLS = 8,      // [8 + c]: next unit of class c (per-class gradient kernels)
   CTR_BIN64 = 32,   // [32 + b]: FP64 gradient sub-list lengths (8 row bins)
   CTR_CLS2 = 40,    // [40 + c]: next unit of class c for the second kernel of a split class
-  CTR_COUNT = 57
+  CTR_BIN64T = 57,  // FP64 gradient list entries of reads beyond 16 rows per lane (the tiled adjoint)
+  CTR_COUNT = 58
 };
 
 // Device view of a genotype batch: region k owns rows row_off[k]..row_off[k+1]
@@ -217,11 +218,23 @@ size_t lds_grad32(int smax);
 size_t lds_grad64(int smax);
 size_t grad_ck_stride(int smax, int fp64);
 hipError_t grad_occupancy(int anchored, int r, int fp64, int variant, int smax, int* blocks_per_cu);
+// The tiled FP64 adjoint of reads beyond 256 rows: tiles of 64 rows, at most
+// ntmax per read (0: no such read, the kernel is not launched); a workspace
+// of ws_stride doubles per block (grad_tile_ws), blocks <= the partial
+// gradient rows.
+struct GradTiled {
+  int ntmax, blocks;
+  double* ws;
+  size_t ws_stride;
+};
+size_t grad_tile_ws(int smax, int ntmax);
+hipError_t grad_tiled_occupancy(int anchored, int smax, int* blocks_per_cu);
 hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, const QTab<float>* tab32,
                        const QTab<double>* tab64, const GradTab* gt, int anchored, int smax,
                        const int* blocks32, const int* variant32, const int* blocks64, int max_blocks, float* ck32,
                        double* ck64, PairRef* bins, int64_t bin_cap, const double* weights, double* out,
-                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls);
+                       double* gpart, double* grad, hipStream_t st, hipEvent_t* ev, hipEvent_t* ev_cls,
+                       const GradTiled* tiled);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 // scratch of the no-allocation forward (vdhmm_noalloc.hip): device address of
 // the library's static array on device `dev`, and its size

@@ -995,101 +995,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_t(ANCH, 
 // 2^CEXP64 before the next tile, so any read length stays in range: a tile
 // decays by at most 64 ln(4/x) nats along the insertion path.
 
-// One tile's sweep of the FP64 forward.  FIRST: the boundary row is the
-// model's start (free starts, or the anchored origin); otherwise it comes
-// from the previous tile's last row, pre-scaled by that tile's
-// renormalisation as it is loaded (16 columns per vector load, one chunk
-// ahead, handed down the lanes by DPP row_shl:1).  LAST: the tile holds the
-// read's last row and accumulates the terminal; otherwise lane 15 stores the
-// tile's last row and its maximum.  Steps before the shortest haplotype's end
-// run without the end-of-haplotype test; both loops advance two steps per
-// iteration so the loop-carried state stays in place.
-template <bool ANCH, bool FIRST, bool LAST>
-__device__ __forceinline__ void tile64(const GroupPair& gp, const uint16_t* cl, int tmain, int tend,
-                                       const double (&gd)[R64], const double (&bd)[R64], const double (&mu)[R64],
-                                       const double (&be)[R64], const uint32_t (&rc)[R64], double gam,
-                                       double gam_last, double xx, double sc1, double sc2,
-                                       const double* __restrict__ inP,
-                                       const double* __restrict__ inQ, double* __restrict__ oP,
-                                       double* __restrict__ oQ, double& acc, double& rmax, double& latch,
-                                       double& bmax) {
-  constexpr int R = R64;
-  const int l = threadIdx.x & 15;
-  double M[R], D[R], Q[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0.0;
-  double Plast = 0.0, Xprev = 0.0;
-  const double C = 0x1p500;
-  double bP = 0.0, bQ = 0.0, nP = 0.0, nQ = 0.0;
-  if (!FIRST) {
-    if (l - 1 <= gp.m_max) {
-      bP = inP[l + 16] * sc1 * sc2;
-      bQ = inQ[l + 16] * sc1 * sc2;
-    }
-    if (l + 15 <= gp.m_max) {
-      nP = inP[l + 32] * sc1 * sc2;
-      nQ = inQ[l + 32] * sc1 * sc2;
-    }
-  }
-  auto step = [&](int t, auto tail) {
-    constexpr bool TAIL = decltype(tail)::value;
-    const uint32_t code = cl[t];
-    double pinj, qinj;
-    if (FIRST) {
-      if (ANCH) {
-        pinj = qinj = t == 0 ? C : 0.0;
-      } else {
-        pinj = (t == 0 || t - 1 > gp.m) ? 0.0 : C;
-        qinj = t == 0 ? 0.0 : C;
-      }
-    } else {
-      if (t > 0 && (t & 15) == 0) {
-        bP = nP;
-        bQ = nQ;
-        nP = nQ = 0.0;
-        if (t + l + 15 <= gp.m_max) {
-          nP = inP[t + l + 32] * sc1 * sc2;
-          nQ = inQ[t + l + 32] * sc1 * sc2;
-        }
-      }
-      pinj = bP;
-      qinj = bQ;
-      bP = dpp_shl1(0.0, bP);
-      bQ = dpp_shl1(0.0, bQ);
-    }
-    const double Pin = dpp_shr1(pinj, Plast);
-    const double Xn = dpp_shr1(qinj, Q[R - 1]);
-    Plast = adiag_step<double, R, TAIL>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
-    Xprev = Xn;
-    if (LAST) {
-      if (ANCH) {
-        rmax = fmax(rmax, Q[R - 1]);
-        latch = (t - l - 1 == gp.m) ? Q[R - 1] : latch;
-      } else {
-        acc += Plast;
-      }
-    } else {
-      if (l == GROUP - 1) {
-        oP[t + 1] = Plast;  // column t-16 -> slot t+1
-        oQ[t + 1] = Q[R - 1];
-      }
-      bmax = fmax(bmax, fmax(Plast, Q[R - 1]));
-    }
-  };
-  using head_t = std::integral_constant<bool, false>;
-  using tail_t = std::integral_constant<bool, true>;
-  int t = 0;
-  for (; t + 1 < tmain; t += 2) {
-    step(t, head_t{});
-    step(t + 1, head_t{});
-  }
-  if (t < tmain) step(t++, head_t{});
-  for (; t + 1 <= tend; t += 2) {
-    step(t, tail_t{});
-    step(t + 1, tail_t{});
-  }
-  if (t <= tend) step(t, tail_t{});
-}
+// One tile's sweep of the FP64 forward: tile64 in vdhmm_device.h (shared
+// with the tiled FP64 adjoint).
 
 template <bool ANCH>
 __device__ __forceinline__ void unit64(const BatchDev& b, const PlanDev& p, const QTab<double>* __restrict__ tab,
