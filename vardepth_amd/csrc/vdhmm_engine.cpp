@@ -146,6 +146,12 @@ struct Engine {
   DevBuf qE, tab32, tab64, gtab;
   // gradient scratch: checkpoints, per-block partial gradients, staged weights / gradient
   DevBuf ck32, ck64, gpart, bins64, in_w, in_grad, ws64t;
+  // scalar drop-ins: one packed upload (inputs, pair list, exponent sum) and
+  // the outputs, staged through one pinned host buffer
+  DevBuf in_fast;
+  void* pinned = nullptr;
+  size_t pinned_cap = 0;
+  int32_t E_host[64] = {};  // FP64 exponents of the parameters on the device
   vd_params params_on_device;
   bool have_params = false;
   int occ32[2][17][2] = {}, occ64[2] = {0, 0};  // [anchored][class][unroll 2 / 4]
@@ -312,6 +318,7 @@ int upload_params(const vd_params& p, hipStream_t st) {
   build_tab(p, t32, E);
   build_tab(p, t64, E);
   build_grad_tab(p, gt);
+  std::memcpy(g.E_host, E, sizeof(E));
   HIPCHK(g.qE.ensure(sizeof(E)));
   HIPCHK(g.tab32.ensure(sizeof(t32)));
   HIPCHK(g.tab64.ensure(sizeof(t64)));
@@ -1325,16 +1332,95 @@ uint64_t trie_cells(const int* rn, int nr, const char* const* haps, const int* h
   return cells;
 }
 
+// One read against `count` haplotypes in FP64 without the batch planner: the
+// read, its qualities, the haplotypes, their offsets, the pair list and the
+// read's exponent sum go up in one copy from a pinned buffer, one launch of
+// k_scalar64 (a unit of 4 pairs per block) computes them, and the outputs come
+// back in one copy.  The values equal the batched FP64 path's bit for bit.
+int one_read_fast_locked(const char* read, const unsigned char* qual, int n, const char* const* haps,
+                         const int* lengths, int count, double gap, double ext, int mode, double* out) {
+  if (count <= 0) return 0;
+  if (mode < 0 || mode > 2) return fail("vdhmm: mode must be 0, 1 or 2");
+  if (n <= 0) {
+    for (int k = 0; k < count; ++k) out[k] = -INFINITY;
+    return 0;
+  }
+  vd_params p;
+  vd_params_default(&p, gap, ext);
+  if (int rc = check_params(p)) return rc;
+  int max_m = 0;
+  size_t hap_bytes = 0;
+  for (int k = 0; k < count; ++k) {
+    if (lengths[k] < 0) return fail("vdhmm: negative haplotype length");
+    max_m = std::max(max_m, lengths[k]);
+    hap_bytes += (size_t)lengths[k];
+  }
+  if (int rc = engine_init()) return rc;
+  if (int rc = check_hap_len(max_m, 0)) return rc;
+  hipStream_t st = g.own_stream;
+  StreamOrder order(st);
+  if (int rc = upload_params(p, st)) return rc;
+  // payload: [read_off 2][hap_off count+1][pairs count][read_exp][read n][qual n][haps]
+  auto up8 = [](size_t x) { return (x + 15) & ~size_t(15); };
+  const size_t o_roff = 0, o_hoff = up8(o_roff + 2 * 8), o_pairs = up8(o_hoff + (size_t)(count + 1) * 8),
+               o_exp = up8(o_pairs + (size_t)count * sizeof(vd::PairRef)), o_read = up8(o_exp + 4),
+               o_qual = up8(o_read + n), o_hap = up8(o_qual + n), bytes = up8(o_hap + hap_bytes + 1),
+               o_out = bytes, total = bytes + (size_t)count * 8;
+  if (g.pinned_cap < total) {
+    if (g.pinned) HIPCHK(hipHostFree(g.pinned));
+    g.pinned = nullptr;
+    g.pinned_cap = 0;
+    const size_t want = std::max<size_t>(total + total / 2, 1 << 16);
+    HIPCHK(hipHostMalloc(&g.pinned, want, hipHostMallocDefault));
+    g.pinned_cap = want;
+  }
+  char* h = static_cast<char*>(g.pinned);
+  int64_t* roff = reinterpret_cast<int64_t*>(h + o_roff);
+  roff[0] = 0;
+  roff[1] = n;
+  int64_t* hoff = reinterpret_cast<int64_t*>(h + o_hoff);
+  hoff[0] = 0;
+  vd::PairRef* pairs = reinterpret_cast<vd::PairRef*>(h + o_pairs);
+  char* hp = h + o_hap;
+  for (int k = 0; k < count; ++k) {
+    if (lengths[k] > 0) std::memcpy(hp + hoff[k], haps[k], (size_t)lengths[k]);
+    hoff[k + 1] = hoff[k] + std::max(lengths[k], 0);
+    pairs[k] = vd::PairRef{0, k, k};
+  }
+  int32_t e = 0;
+  for (int i = 0; i < n; ++i) e += g.E_host[std::min<int>(qual[i], 63)];
+  *reinterpret_cast<int32_t*>(h + o_exp) = e;
+  std::memcpy(h + o_read, read, (size_t)n);
+  std::memcpy(h + o_qual, qual, (size_t)n);
+  HIPCHK(g.in_fast.ensure(total));
+  char* d = g.in_fast.as<char>();
+  HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+  const int smax = ((max_m + 48) + 7) & ~7;
+  const int blocks = (count + vd::PAIRS_PER_UNIT - 1) / vd::PAIRS_PER_UNIT;
+  HIPCHK(g.ws64.ensure((size_t)blocks * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
+  HIPCHK(g.ctr.ensure(vd::CTR_COUNT * 8));
+  const vd::BatchDev b{reinterpret_cast<const uint8_t*>(d + o_read), reinterpret_cast<const uint8_t*>(d + o_qual),
+                       reinterpret_cast<const int64_t*>(d + o_roff), reinterpret_cast<const uint8_t*>(d + o_hap),
+                       reinterpret_cast<const int64_t*>(d + o_hoff), nullptr, nullptr, nullptr, 1, 1, count};
+  vd::PlanDev pd{};
+  pd.read_exp = reinterpret_cast<int32_t*>(d + o_exp);
+  pd.ctr = g.ctr.as<unsigned long long>();
+  double* dout = reinterpret_cast<double*>(d + o_out);
+  HIPCHK(vd::launch_scalar64(b, pd, reinterpret_cast<const vd::PairRef*>(d + o_pairs), count,
+                             g.tab64.as<vd::QTab<double>>(), mode == VD_MODE_ANCHORED ? 1 : 0, smax, g.ws64.as<double>(),
+                             dout, st));
+  HIPCHK(hipMemcpyAsync(h + o_out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::memcpy(out, h + o_out, (size_t)count * 8);
+  return 0;
+}
+
 double single(const char* read, const unsigned char* qual, int n, const char* hap, int m, double gap, double ext,
               int mode) {
   if (n <= 0 || m <= 0) return -INFINITY;
-  HostBatch hb;
-  hb.add_read(read, qual, n);
-  hb.add_hap(hap, m);
-  hb.close_region();
   double v = NAN;
   EagerLock lk;
-  if (run_host(hb, gap, ext, mode, VD_PREC_FP64, &v)) {
+  if (one_read_fast_locked(read, qual, n, &hap, &m, 1, gap, ext, mode, &v)) {
     std::fprintf(stderr, "%s\n", g_err.c_str());
     return NAN;
   }
@@ -1344,12 +1430,8 @@ double single(const char* read, const unsigned char* qual, int n, const char* ha
 void one_read_batch(const char* read, const unsigned char* qual, int n, const char* const* haps, const int* lengths,
                     int count, double gap, double ext, double* out, int mode) {
   if (count <= 0) return;
-  HostBatch hb;
-  hb.add_read(read, qual, n);
-  for (int k = 0; k < count; ++k) hb.add_hap(haps[k], lengths[k]);
-  hb.close_region();
   EagerLock lk;
-  if (run_host(hb, gap, ext, mode, VD_PREC_FP64, out)) {
+  if (one_read_fast_locked(read, qual, n, haps, lengths, count, gap, ext, mode, out)) {
     std::fprintf(stderr, "%s\n", g_err.c_str());
     for (int k = 0; k < count; ++k) out[k] = NAN;
   }

@@ -210,6 +210,12 @@ hipError_t launch_pipeline(const BatchDev& b, const PlanDev& p, const int32_t* q
 // FP64 list binning (f64_sorted / f64_bins non-null): capacity of the sorted
 // copy = the list's, bins = 2 * F64_BINS_TOTAL counters
 constexpr int F64_BINS_TOTAL = 16 * 32;
+// One read against `count` haplotypes in FP64, one unit of 4 pairs per block
+// (the scalar drop-ins): pairs, ws (4 * 4 * smax doubles per block) and out
+// on the device.
+hipError_t launch_scalar64(const BatchDev& b, const PlanDev& p, const PairRef* pairs, int count,
+                           const QTab<double>* tab64, int anchored, int smax, double* ws, double* out,
+                           hipStream_t st);
 hipError_t launch_ref_pairs(const BatchDev& b, const PairRef* pairs, int count, int cols, const float* err,
                             float gap, float ext, int anchored, float* ws, double* out, hipStream_t st);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,

@@ -1249,6 +1249,62 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   }
 }
 
+// One read against `count` haplotypes in FP64, for the scalar drop-ins: block
+// u runs pairs 4u .. 4u+3 as one unit of the FP64 kernel, with no plan and no
+// work list.  A pair's FP64 result depends on the pair alone, so these values
+// equal the batched FP64 path's bit for bit.  p supplies read_exp and the
+// counters only.
+template <bool ANCH>
+__global__ __launch_bounds__(64) void k_scalar64(BatchDev b, PlanDev p, const PairRef* __restrict__ pairs, int count,
+                                                 const QTab<double>* __restrict__ gtab, int smax,
+                                                 double* __restrict__ wsall, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
+  uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(gtab);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tab);
+    for (int i = threadIdx.x; i < (int)(qtab_lds_bytes<double>() / 4); i += blockDim.x) dst[i] = src[i];
+  }
+  __syncthreads();
+  double* ws = wsall + (size_t)blockIdx.x * 4 * PAIRS_PER_UNIT * smax;
+  const int lane = threadIdx.x & 63, g = lane >> 4;
+  const int idx = blockIdx.x * PAIRS_PER_UNIT + g;
+  GroupPair gp;
+  gp.valid = idx < count;
+  const PairRef pr = pairs[gp.valid ? idx : blockIdx.x * PAIRS_PER_UNIT];
+  gp.read = pr.read;
+  gp.hap = pr.hap;
+  gp.out = pr.out;
+  gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+  gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+  if (gp.valid && gp.m == 0) {
+    if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
+    gp.valid = 0;
+  }
+  const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+  const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+  const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
+  if (mx == 0) return;
+  gp.m_max = mx;
+  gp.m_min = mn;
+  if (!gp.valid) gp.m = mx;
+  unit64<ANCH>(b, p, tab, codes, smax, gp, ws, out);
+}
+
+hipError_t launch_scalar64(const BatchDev& b, const PlanDev& p, const PairRef* pairs, int count,
+                           const QTab<double>* tab64, int anchored, int smax, double* ws, double* out,
+                           hipStream_t st) {
+  const dim3 grid((count + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT);
+  if (anchored)
+    hipLaunchKernelGGL(k_scalar64<true>, grid, dim3(64), lds_fwd64(smax), st, b, p, pairs, count, tab64, smax, ws,
+                       out);
+  else
+    hipLaunchKernelGGL(k_scalar64<false>, grid, dim3(64), lds_fwd64(smax), st, b, p, pairs, count, tab64, smax, ws,
+                       out);
+  return hipGetLastError();
+}
+
 template __global__ void k_fwd64<false>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
 template __global__ void k_fwd64<true>(BatchDev, PlanDev, const QTab<double>*, int, double*, double*);
 
