@@ -368,7 +368,9 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     `fwd_bwd_gcups` / `ratio` count a forward pass plus a gradient call, as an
     autograd step runs them.  One gradient call also returns ln L per pair
     (its first sweep is the forward, and the FP64 adjoint rescues the pairs
-    it flags), so `fused_gcups` / `fused_ratio` count that call alone."""
+    it flags), so `fused_gcups` / `fused_ratio` count that call alone.
+    `training` is config 4's own run: 200 Adam steps on the true-haplotype
+    loss from a perturbed start, timed, with the recovered parameters."""
     import torch
     from vardepth_amd import engine, workloads
     b = workloads.hmm_sampled_regions(seed=41004, regions=regions)
@@ -394,6 +396,31 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     torch.cuda.synchronize()
     grad_ms = (time.perf_counter() - t0) / steps * 1e3
     k32 = float(np.mean([s_["fwd32_ms"] for s_ in sts]))
+    # config 4's training run: the loss -mean ln L over each read's true
+    # haplotype (the true-pair batch), 200 Adam steps from a perturbed start,
+    # each step one forward and one adjoint call through torch.autograd
+    from vardepth_amd import fit as fitmod
+    true_gap, true_ext = 0.01, 0.3
+    err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
+    tp = workloads.true_pairs(b)
+    devt = engine.DeviceBatch(tp)
+    gap0, ext0, err0 = fitmod.perturbed_start(41004, true_gap, true_ext, err_true)
+    fitmod.fit(devt, gap0, ext0, err0, steps=2, report=2)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist = fitmod.fit(devt, gap0, ext0, err0, steps=200, lr=0.05, report=200)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    used = np.arange(10, 41)
+    lerr = lambda e: float(np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used]))))
+    training = {"workload": "true-haplotype pairs of the same regions (loss -mean ln L)", "pairs": tp.npairs,
+                "cells": tp.cells(), "adam_steps": 200, "seconds": round(fit_s, 3),
+                "ms_per_step": round(fit_s / 200 * 1e3, 3),
+                "fwd_bwd_gcups": round(tp.cells() * 200 / fit_s / 1e9, 2),
+                "gap": [round(hist[0]["gap"], 5), round(hist[-1]["gap"], 5), true_gap],
+                "ext": [round(hist[0]["ext"], 4), round(hist[-1]["ext"], 4), true_ext],
+                "mean_abs_log10_err_q10_40": [round(lerr(hist[0]["err"]), 4), round(lerr(hist[-1]["err"]), 4)],
+                "loss": [round(hist[0]["loss"], 4), round(hist[-1]["loss"], 4)]}
     fwd_g = cells / fwd_ms / 1e6
     fb_g = cells / (fwd_ms + grad_ms) / 1e6
     fused_g = cells / grad_ms / 1e6
@@ -402,7 +429,7 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "adjoint_kernel_ms": round(k32, 3), "fwd_gcups": round(fwd_g, 3), "fwd_bwd_gcups": round(fb_g, 3),
             "ratio": round(fb_g / fwd_g, 4), "fused_gcups": round(fused_g, 3),
             "fused_ratio": round(fused_g / fwd_g, 4), "flagged": int(sts[-1]["flagged"]),
-            "autotune": {"forward": tune_f, "adjoint": tune_g}}
+            "autotune": {"forward": tune_f, "adjoint": tune_g}, "training": training}
 
 
 def scalar_bench(pairs: int = 2000, calls: int = 400):
