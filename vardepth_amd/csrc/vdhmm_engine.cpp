@@ -1350,10 +1350,9 @@ int one_read_fast_locked(const char* read, const unsigned char* qual, int n, con
   if (int rc = check_params(p)) return rc;
   int max_m = 0;
   size_t hap_bytes = 0;
-  for (int k = 0; k < count; ++k) {
-    if (lengths[k] < 0) return fail("vdhmm: negative haplotype length");
+  for (int k = 0; k < count; ++k) {  // a negative length is an empty haplotype (-inf), as in the batched path
     max_m = std::max(max_m, lengths[k]);
-    hap_bytes += (size_t)lengths[k];
+    hap_bytes += (size_t)std::max(lengths[k], 0);
   }
   if (int rc = engine_init()) return rc;
   if (int rc = check_hap_len(max_m, 0)) return rc;
