@@ -330,3 +330,36 @@ def test_canonical_batch_equals_scalar_anchored():
     np.testing.assert_array_equal(got[:len(jobs)], np.asarray(expected)[perm])
     ref = [oracle.dense(s.encode(), bytes(q), h.encode(), 1) for s, q, h in jobs]
     close(expected, ref, TOL64)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_scalar_fast_path_equals_batched_fp64(mode):
+    """The one-read entry points (one packed copy, one FP64 unit kernel, no
+    planner) return the batched FP64 forward's values bit for bit: reads of 1
+    to 300 bases (up to 5 tiles of 64 rows), up to 11 haplotypes per call (3
+    blocks of 4), empty haplotypes and negative lengths (-inf)."""
+    from vardepth_amd import engine, workloads
+    rng = np.random.default_rng(60 + mode)
+    lib = native.kernel() if mode == 1 else (native.core() if mode == 0 else native.wide_library())
+    name = "anchored_pairhmm" if mode == 1 else "read_haplotype"
+    for n in (1, 37, 150, 300):
+        read = rng.choice(list(b"ACGT"), n).astype(np.uint8).tobytes()
+        qual = rng.integers(2, 41, n).astype(np.uint8).tobytes()
+        haps = [rng.choice(list(b"ACGTN"), int(m)).astype(np.uint8).tobytes() for m in rng.integers(1, 360, 11)]
+        haps[3] = b""
+        b = workloads.from_regions([([read], [qual], haps)], 0.001, 0.1)
+        want = engine.forward(b, mode=mode, precision="float64")
+        q = (ctypes.c_ubyte * n).from_buffer_copy(qual)
+        one = np.array([getattr(lib, name)(read, q, n, h, len(h), 0.001, 0.1) for h in haps])
+        np.testing.assert_array_equal(one, want)
+        if mode != 1:
+            hp = (ctypes.c_char_p * len(haps))(*haps)
+            lens = (ctypes.c_int * len(haps))(*[len(h) for h in haps])
+            out = np.empty(len(haps))
+            lib.read_haplotype_batch(read, q, n, hp, lens, len(haps), 0.001, 0.1,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            np.testing.assert_array_equal(out, want)
+            lens[5] = -3  # a negative length is an empty haplotype
+            lib.read_haplotype_batch(read, q, n, hp, lens, len(haps), 0.001, 0.1,
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+            assert out[5] == -np.inf and np.array_equal(np.delete(out, 5), np.delete(want, 5))
