@@ -389,6 +389,20 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
         dev.forward(mode=mode)
     torch.cuda.synchronize()
     fwd_ms = (time.perf_counter() - t0) / steps * 1e3
+    # the same forward without haplotype prefix sharing (the adjoint sweeps
+    # every pair in full): the like-for-like denominator
+    from vardepth_amd import _lib
+    prev = _lib.lib().vd_prefix_sharing(0)
+    try:
+        dev.forward(mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.forward(mode=mode)
+        torch.cuda.synchronize()
+        fwd_full_ms = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        _lib.lib().vd_prefix_sharing(prev)
     sts = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -428,7 +442,9 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "pairs": b.npairs, "cells": cells, "fwd_ms": round(fwd_ms, 3), "adjoint_ms": round(grad_ms, 3),
             "adjoint_kernel_ms": round(k32, 3), "fwd_gcups": round(fwd_g, 3), "fwd_bwd_gcups": round(fb_g, 3),
             "ratio": round(fb_g / fwd_g, 4), "fused_gcups": round(fused_g, 3),
-            "fused_ratio": round(fused_g / fwd_g, 4), "flagged": int(sts[-1]["flagged"]),
+            "fused_ratio": round(fused_g / fwd_g, 4),
+            "fwd_unshared_ms": round(fwd_full_ms, 3), "fused_ratio_vs_unshared_fwd": round(fwd_full_ms / grad_ms, 4),
+            "flagged": int(sts[-1]["flagged"]),
             "autotune": {"forward": tune_f, "adjoint": tune_g}, "training": training}
 
 
