@@ -410,6 +410,23 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     torch.cuda.synchronize()
     grad_ms = (time.perf_counter() - t0) / steps * 1e3
     k32 = float(np.mean([s_["fwd32_ms"] for s_ in sts]))
+    # the same call with weight 1 on each read's true haplotype and 0 elsewhere
+    # (a loss on the true pairs of the full matrix): units whose four weights
+    # are zero write their ln L and skip the backward sweeps
+    oo = b.out_off()
+    w = np.zeros(b.npairs)
+    for k in range(b.nregions):
+        r0, r1, h0 = b.region_reads[k], b.region_reads[k + 1], b.region_haps[k]
+        H = b.region_haps[k + 1] - h0
+        w[oo[k] + np.arange(r1 - r0) * H + (b.read_true_hap[r0:r1] - h0)] = 1.0
+    wdev = torch.from_numpy(w).to("cuda")
+    dev.grad(mode=mode, weights=wdev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.grad(mode=mode, weights=wdev)
+    torch.cuda.synchronize()
+    truth_w_ms = (time.perf_counter() - t0) / steps * 1e3
     # config 4's training run: the loss -mean ln L over each read's true
     # haplotype (the true-pair batch), 200 Adam steps from a perturbed start,
     # each step one forward and one adjoint call through torch.autograd
@@ -444,6 +461,7 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
             "ratio": round(fb_g / fwd_g, 4), "fused_gcups": round(fused_g, 3),
             "fused_ratio": round(fused_g / fwd_g, 4),
             "fwd_unshared_ms": round(fwd_full_ms, 3), "fused_ratio_vs_unshared_fwd": round(fwd_full_ms / grad_ms, 4),
+            "truth_weights_adjoint_ms": round(truth_w_ms, 3),
             "flagged": int(sts[-1]["flagged"]),
             "autotune": {"forward": tune_f, "adjoint": tune_g}, "training": training}
 

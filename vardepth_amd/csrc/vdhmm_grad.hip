@@ -229,6 +229,13 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
     const int64_t pick = g == 0 ? sv[0] : g == 1 ? sv[1] : g == 2 ? sv[2] : sv[3];
     seed = __builtin_bit_cast(double, pick);
     flag = (g == 0 ? fv[0] : g == 1 ? fv[1] : g == 2 ? fv[2] : fv[3]) != 0;
+    // a unit whose four seeds are all zero (weights 0, flagged or invalid
+    // pairs) contributes nothing to the gradient: its outputs are written,
+    // the backward sweeps are skipped (the decision is the wave's: scalars)
+    if ((sv[0] | sv[1] | sv[2] | sv[3]) == 0) {
+      __syncthreads();  // codes reused by the next unit
+      return;
+    }
   }
   const T tseed = (T)seed;
 
@@ -911,6 +918,10 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
     }
     seed = __shfl(seed, g * GROUP + GROUP - 1);
     flag = __shfl((int)flag, g * GROUP + GROUP - 1) != 0;
+    if (!__builtin_amdgcn_ballot_w64(seed != 0.0)) {  // no pair of the unit contributes: outputs only
+      __syncthreads();
+      continue;
+    }
     // ---- phase B: the tiles last to first
     for (int k = lane; k < 4 * smax; k += 64) {  // both adjoint buffers of this group start at zero
       const int s_ = k / smax, slot = k % smax;
