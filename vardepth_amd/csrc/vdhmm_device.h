@@ -161,6 +161,48 @@ __host__ __device__ __forceinline__ int64_t bucket_of(int c, int64_t k, int64_t 
   return class_major ? (int64_t)(NCLS - 1 - c) * nregions + k : k * NCLS + c;
 }
 
+// Packed plans (pack_pairs): pair pi of the class-major pair sequence (valid:
+// below the class's end) -> the lane's group pair, with the unit's extent
+// over its four groups; returns -1 when every group of the unit is empty.
+// Pairs with an empty haplotype get -inf here and are marked invalid.
+__device__ __forceinline__ int decode_pair32(const BatchDev& b, const PlanDev& p, int64_t pi, bool valid,
+                                             GroupPair& gp, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  gp.valid = valid;
+  gp.read = 0;
+  gp.hap = 0;
+  gp.out = 0;
+  gp.n = 0;
+  gp.m = 0;
+  if (valid) {
+    const int bkt = p.unit_bucket[pi];
+    const int64_t j = pi - p.unit_cnt[bkt];
+    const int64_t k = bkt % b.nregions;
+    const int c = NCLS - 1 - (int)(bkt / b.nregions);
+    const int64_t r0 = b.region_reads[k], h0 = b.region_haps[k];
+    const int64_t H = b.region_haps[k + 1] - h0;
+    const int32_t* co = p.class_off + k * (NCLS + 1);
+    const int64_t s = j / H, h = j % H;
+    gp.read = p.read_perm[r0 + co[c] + s];
+    gp.hap = h0 + h;
+    gp.out = b.out_off[k] + (gp.read - r0) * H + h;
+    gp.n = (int)(b.read_off[gp.read + 1] - b.read_off[gp.read]);
+    gp.m = (int)(b.hap_off[gp.hap + 1] - b.hap_off[gp.hap]);
+    if (gp.m == 0) {
+      if ((lane & 15) == 0) out[gp.out] = -__builtin_inf();
+      gp.valid = 0;
+    }
+  }
+  const int mx0 = gp.valid ? gp.m : 0, mn0 = gp.valid ? gp.m : 0x7fffffff;
+  const int mx = max(max(lane16(mx0, 0), lane16(mx0, 1)), max(lane16(mx0, 2), lane16(mx0, 3)));
+  const int mn = min(min(lane16(mn0, 0), lane16(mn0, 1)), min(lane16(mn0, 2), lane16(mn0, 3)));
+  if (mx == 0) return -1;
+  gp.m_max = mx;
+  gp.m_min = mn;
+  if (!gp.valid) gp.m = mx;  // an idle group: read 0, no rows, nothing written
+  return 0;
+}
+
 // FP32 unit u -> the lane's group pair; returns the unit's class (rows per
 // lane), or -1 when every group of the unit is empty.  Pairs with an empty
 // haplotype get -inf here and are marked invalid.

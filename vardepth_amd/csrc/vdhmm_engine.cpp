@@ -366,7 +366,8 @@ int plan_buffers(const vd_batch* db, int64_t npairs) {
   HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
   HIPCHK(g.read_perm.ensure(nreads * 8));
   HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
-  HIPCHK(g.unit_bucket.ensure(max_units * 4));
+  // a unit map, or with packed pairs (the gradients) a bucket per FP32 pair
+  HIPCHK(g.unit_bucket.ensure(std::max<int64_t>(max_units, npairs + 1) * 4));
   HIPCHK(g.f64_list.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
   HIPCHK(g.ctr.ensure(vd::CTR_COUNT * 8));
   return 0;
@@ -784,7 +785,8 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
   HIPCHK(g.gpart.ensure((size_t)max_blocks * vd::NGRAD * sizeof(double)));
   HIPCHK(g.bins64.ensure((size_t)9 * bin_cap * sizeof(vd::PairRef)));  // 8 row bins + the tiled bin
   const vd::BatchDev b = batch_dev(db);
-  const vd::PlanDev p = plan_dev(npairs, 1);
+  vd::PlanDev p = plan_dev(npairs, 1);
+  p.pack_pairs = 1;  // pair units may span regions of a class: no idle lanes for small regions
   HIPCHK(vd::launch_grad(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(), g.tab64.as<vd::QTab<double>>(),
                          g.gtab.as<vd::GradTab>(), anchored, smax, blocks32, variant32, blocks64, max_blocks,
                          g.ck32.as<float>(), g.ck64.as<double>(), g.bins64.as<vd::PairRef>(), bin_cap, dweights,
@@ -811,7 +813,7 @@ int grad_device_locked(const vd_batch* db, const vd_params* params, int mode, in
     stats->fwd32_ms = b2;
     stats->fwd64_ms = c3;
     stats->pairs = (uint64_t)npairs;
-    stats->fp32_units = c[vd::CTR_NUNITS];
+    stats->fp32_units = (c[vd::CTR_NUNITS] + vd::PAIRS_PER_UNIT - 1) / vd::PAIRS_PER_UNIT;  // packed: FP32 pairs / 4
     stats->flagged = c[vd::CTR_FLAGGED];
     stats->fp64_pairs = std::min<unsigned long long>(c[vd::CTR_F64], (unsigned long long)std::max<int64_t>(npairs, 1));
     stats->fp64_bad = c[vd::CTR_F64BAD];

@@ -423,9 +423,12 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
                                             const GradTab* __restrict__ gt, int smax, float* __restrict__ ckall,
                                             size_t ck_stride, const double* __restrict__ weights,
                                             double* __restrict__ out, double* __restrict__ gpart) {
+  // the class's units: [lo, hi) of the unit sequence, or, with packed pairs,
+  // the pairs [lo, hi) in units of 4 that may span regions
   const unsigned long long lo = (unsigned long long)p.unit_cnt[bucket_of(R, 0, b.nregions, 1)];
   const unsigned long long hi = (unsigned long long)p.unit_cnt[bucket_of(R - 1, 0, b.nregions, 1)];
   if (lo >= hi) return;  // no read of this class: the block's partial row stays as it is
+  const unsigned long long nunits = p.pack_pairs ? (hi - lo + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT : hi - lo;
   extern __shared__ __align__(16) unsigned char lds[];
   grad_lds_init<float>(gtab, qtab_lds_bytes<float>(), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
@@ -437,11 +440,16 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
   double acc[2] = {0.0, 0.0};
   for (;;) {
     unsigned long long u = 0;
-    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
+    if (lane == 0) u = atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
     u = first_lane(u);
-    if (u >= hi) break;
+    if (u >= nunits) break;
     GroupPair gp;
-    if (decode_unit32(b, p, u, gp, out) != R) continue;
+    if (p.pack_pairs) {
+      const unsigned long long pi = lo + u * PAIRS_PER_UNIT + (lane >> 4);
+      if (decode_pair32(b, p, (int64_t)pi, pi < hi, gp, out) < 0) continue;
+    } else if (decode_unit32(b, p, lo + u, gp, out) != R) {
+      continue;
+    }
     unit_grad<float, R, ANCH, GK>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false);
   }
   grad_flush(acc[0], acc[1], lds_de, gpart);

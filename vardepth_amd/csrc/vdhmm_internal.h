@@ -128,6 +128,11 @@ struct PlanDev {
                             // with a shared suffix, the prefix-only kernel the others
   int32_t* trie_sfx;        // [nregions] common suffix S (0: none)
   float* sfx_ws;            // [block][64 lanes][84]: adjoint weights, boundary mass, the column m - S state
+  // pair units packed across regions (class-major plans of the gradients): the
+  // per-bucket counts are pairs, not units, and unit_bucket maps every pair of
+  // a class to its bucket, so a unit of 4 consecutive pairs of a class may
+  // span regions and small regions leave no lanes idle
+  int pack_pairs;
 };
 
 constexpr int TRIE_SLOTS = 32;  // distinct fork columns kept per region (later forks are swept in full)

@@ -130,8 +130,9 @@ __global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
       const bool tc = (p.trie >> c) & 1;
       int64_t units = 0;
       if (c >= 1 && c <= MAX_R32 && H > 0)
-        units = tc ? ((int64_t)cnt[c] + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT * ((H + p.trie_chunk - 1) / p.trie_chunk)
-                   : ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
+        units = p.pack_pairs ? (int64_t)cnt[c] * H
+                : tc ? ((int64_t)cnt[c] + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT * ((H + p.trie_chunk - 1) / p.trie_chunk)
+                     : ((int64_t)cnt[c] * H + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
       p.unit_cnt[bucket_of(c, k, b.nregions, p.class_major)] = units;
     }
     __syncthreads();
