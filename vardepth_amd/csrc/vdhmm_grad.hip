@@ -65,23 +65,25 @@ struct GradScale<double> {
 };
 
 // One forward step that also records each row's diagonal source (qu) and new
-// D^ (dd).  Emission is zero past the haplotype end at every step (codes past
-// the end only occur in the tail, as in the forward kernels).
-template <typename T, int R>
+// D^ (dd).  Emission is zero past the haplotype end (TAIL steps: codes past
+// an end only occur at steps >= m_min + 2, as in the forward kernels).  FIRST:
+// the state was just restored from a checkpoint (M = x D^ + M^, D = 0), so
+// the new D^ is M itself.
+template <typename T, int R, bool TAIL = true>
 __device__ __forceinline__ T gstep(T (&M)[R], T (&D)[R], T (&Q)[R], const T (&gd)[R], const T (&bd)[R],
                                    const T (&mu)[R], const T (&be)[R], const uint32_t (&rc)[R], T gam, T gam_last,
-                                   T xx, T Pin, T Qin, uint32_t code, T (&qu)[R], T (&dd)[R]) {
+                                   T xx, T Pin, T Qin, uint32_t code, T (&qu)[R], T (&dd)[R], bool first = false) {
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    D[r] = fmaT(xx, D[r], M[r]);
+    D[r] = first ? M[r] : fmaT(xx, D[r], M[r]);
     dd[r] = D[r];
   }
-  const bool post = code == SENT_POST;
+  const bool post = TAIL && code == SENT_POST;
 #pragma unroll
   for (int r = R - 1; r >= 0; --r) {
     const T qup = r == 0 ? Qin : Q[r - 1];
     T e = rc[r] == code ? gd[r] : bd[r];
-    e = post ? T(0) : e;
+    if (TAIL) e = post ? T(0) : e;
     qu[r] = qup;
     M[r] = e * qup;
   }
@@ -110,11 +112,13 @@ template <typename T, int R, bool ANCH, int GK>
 __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, const QTab<T>* __restrict__ tab,
                                           const GradTab* __restrict__ gt, uint16_t* codes, int smax,
                                           const GroupPair gp, T* __restrict__ ck, const double* __restrict__ weights,
-                                          double* __restrict__ out, double* acc, double* lds_de, bool rescue) {
+                                          double* __restrict__ out, double* acc, double* lds_de, bool rescue,
+                                          T* ckl = nullptr) {
   constexpr int CKW = 2 * R + 2;
   const int lane = threadIdx.x & 63, l = lane & 15;
   const int tend = gp.m_max + 16;
-  fill_codes(b, gp, codes, smax, tend + 18);
+  // codes up to the padded last step of the last checkpoint block
+  fill_codes(b, gp, codes, smax, tend + 16 + GK * NSUB);
   T gd[R], bd[R], mu[R], be[R];
   uint32_t rc[R];
   bool pass[R];  // pass-through row above the read (structural mu = beta = 1)
@@ -152,35 +156,50 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   const uint16_t* cl = codes + (lane >> 4) * smax + 16 - l;
   __syncthreads();
 
-  // ---- sweep 1: forward with checkpoints
+  // ---- sweep 1: forward with checkpoints.  Steps before tmain see no column
+  // past any haplotype end; they run in aligned groups of CK_STEPS without the
+  // end test, the rest with it.
   T M[R], D[R], Q[R], qu[R], dd[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) M[r] = D[r] = Q[r] = 0;
   T Plast = 0, Xprev = 0, term_acc = 0, rmax = 0, latch = 0;
   constexpr int CK_STEPS = GK * NSUB;
-  for (int t = 0; t <= tend; ++t) {
-    if (t % CK_STEPS == 0) {
-      T* c = ck + (size_t)(t / CK_STEPS) * CKW * 64 + lane;
+  const int tmain = gp.m_min + 2;
+  auto ck_store = [&](int t) {
+    T* c = ck + (size_t)(t / CK_STEPS) * CKW * 64 + lane;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        c[(size_t)r * 64] = fmaT(xx, D[r], M[r]);
-        c[(size_t)(R + r) * 64] = Q[r];
-      }
-      c[(size_t)(2 * R) * 64] = Plast;
-      c[(size_t)(2 * R + 1) * 64] = Xprev;
+    for (int r = 0; r < R; ++r) {
+      c[(size_t)r * 64] = fmaT(xx, D[r], M[r]);
+      c[(size_t)(R + r) * 64] = Q[r];
     }
+    c[(size_t)(2 * R) * 64] = Plast;
+    c[(size_t)(2 * R + 1) * 64] = Xprev;
+  };
+  auto fwd_step = [&](int t, auto tail_c) {
+    constexpr bool TL = decltype(tail_c)::value;
     T pinj, qinj;
     injections<T, ANCH>(t, gp.m, C, pinj, qinj);
     const T Pin = dpp_shr1(pinj, Plast);
     const T Xn = dpp_shr1(qinj, Q[R - 1]);
-    Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
+    Plast = gstep<T, R, TL>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
     Xprev = Xn;
     if (ANCH) {
       rmax = fmax(rmax, Q[R - 1]);
-      if (t - l - 1 == gp.m) latch = Q[R - 1];
+      // lane 15 reaches column m at step m + 16 >= tmain
+      if (TL && t - l - 1 == gp.m) latch = Q[R - 1];
     } else {
       term_acc += Plast;
     }
+  };
+  int t1 = 0;
+  for (; t1 + CK_STEPS <= tmain && t1 + CK_STEPS - 1 <= tend; t1 += CK_STEPS) {
+    ck_store(t1);
+#pragma unroll
+    for (int k = 0; k < CK_STEPS; ++k) fwd_step(t1 + k, std::false_type{});
+  }
+  for (; t1 <= tend; ++t1) {
+    if (t1 % CK_STEPS == 0) ck_store(t1);
+    fwd_step(t1, std::true_type{});
   }
 
   // ---- terminal, flag and seed (lane 15 of each group holds the pair's terminal)
@@ -239,87 +258,83 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   }
   const T tseed = (T)seed;
 
-  // ---- sweep 2: backward, block by block
+  // ---- sweep 2: backward, block by block.  FP32: every block runs its
+  // CK_STEPS steps (the last one's steps past tend see columns past every
+  // haplotype end, where the forward values on the P chain and every gradient
+  // term are zero), so its register blocks are straight-line code without
+  // step guards, and the blocks below tmain skip the end test.  FP64 keeps
+  // guarded steps: the straight-line form needs more than its 512 registers.
   T EMp[R], Dbr[R], Er[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) EMp[r] = Dbr[r] = Er[r] = 0;
   T U = 0, V = 0, UL = 0, VL = 0, Tg = 0, Tl = 0, Tx = 0, Ilast = 0;
-  // the anchored terminal of a pair whose haplotype is the unit's longest
-  // enters before the first backward step
-  T Eprev = (ANCH && l == GROUP - 1 && tend - 16 == gp.m) ? tseed : T(0);
   const T Pseed = ANCH ? T(0) : tseed;
   const int nblk = tend / CK_STEPS + 1;
-  // Where the kernel already runs one wave per SIMD, the previous block's
-  // checkpoint is loaded while the current block is processed (the extra
-  // registers cost no occupancy there, and nothing else would hide the load).
-  constexpr bool PREF = sizeof(T) == 4 ? false : R >= 8;
-  T cur[CKW], ckn[PREF ? CKW : 1];
-  if (PREF) {
-    const T* c = ck + (size_t)(nblk - 1) * CKW * 64 + lane;
-#pragma unroll
-    for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
-  }
-  for (int blk = nblk - 1; blk >= 0; --blk) {
-    if (PREF) {
-#pragma unroll
-      for (int w = 0; w < CKW; ++w) cur[w] = ckn[w % (PREF ? CKW : 1)];
-      if (blk > 0) {
-        const T* c = ck + (size_t)(blk - 1) * CKW * 64 + lane;
-#pragma unroll
-        for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
-      }
-    } else {
+  if constexpr (sizeof(T) == 4) {
+    const int tlast = nblk * CK_STEPS - 1;
+    // the anchored terminal enters lane 15 through the reverse carry at step
+    // m + 17; a pair with m + 16 = tlast has it before the first backward step
+    T Eprev = (ANCH && l == GROUP - 1 && tlast - 16 == gp.m) ? tseed : T(0);
+    // checkpoint blocks are staged through LDS (ckl): the memory unit copies
+    // block blk - 1's checkpoint straight into LDS while block blk runs, so
+    // the wave never waits on HBM at a block start
+    T cur[CKW];
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    auto stage = [&](int blk) {
       const T* c = ck + (size_t)blk * CKW * 64 + lane;
 #pragma unroll
-      for (int w = 0; w < CKW; ++w) cur[w] = c[(size_t)w * 64];
-    }
-    const int tb = blk * CK_STEPS;
-    for (int sub = NSUB - 1; sub >= 0; --sub) {
-      const int t0 = tb + sub * GK;
-      if (t0 > tend) continue;
-      const int cnt = min(GK, tend + 1 - t0);
-      // state before step tb: M = x D^ + M^ and D = 0 make the first step's
-      // D update reproduce D^(tb) exactly
+      for (int w = 0; w < CKW; ++w) __builtin_amdgcn_global_load_lds(c + (size_t)w * 64, (lds_ptr)(ckl + w * 64), 4, 0, 0);
+    };
+    stage(nblk - 1);
+    // one checkpoint block, its NSUB register blocks last to first
+    auto bwd_block = [&](int tb, auto tail_c) {
+      constexpr bool TL = decltype(tail_c)::value;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        M[r] = cur[r];
-        D[r] = 0;
-        Q[r] = cur[R + r];
-      }
-      Plast = cur[2 * R];
-      Xprev = cur[2 * R + 1];
-      for (int t = tb; t < t0; ++t) {  // steps before the register block, not stored
-        T pinj, qinj;
-        injections<T, ANCH>(t, gp.m, C, pinj, qinj);
-        const T Pin = dpp_shr1(pinj, Plast);
-        const T Xn = dpp_shr1(qinj, Q[R - 1]);
-        Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
-        Xprev = Xn;
-      }
-      T sQ[GK][R], sD[GK][R], sPin[GK];
+      for (int sub = NSUB - 1; sub >= 0; --sub) {
+        // the register blocks are independent until their backward steps: keep
+        // the scheduler from overlapping them (that doubles the live values)
+        __builtin_amdgcn_sched_barrier(0);
+        const int t0 = tb + sub * GK;
+        T Mr[R], Dr[R], Qr[R], qx[R], dx[R];
 #pragma unroll
-      for (int s = 0; s < GK; ++s) {
-        if (s < cnt) {
+        for (int r = 0; r < R; ++r) {
+          Mr[r] = cur[r];
+          Dr[r] = 0;
+          Qr[r] = cur[R + r];
+        }
+        T Pl = cur[2 * R], Xp = cur[2 * R + 1];
+#pragma unroll
+        for (int k = 0; k < sub * GK; ++k) {  // steps before the register block, not stored
+          const int t = tb + k;
+          T pinj, qinj;
+          injections<T, ANCH>(t, gp.m, C, pinj, qinj);
+          const T Pin = dpp_shr1(pinj, Pl);
+          const T Xn = dpp_shr1(qinj, Qr[R - 1]);
+          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, cl[t], qx, dx, k == 0);
+          Xp = Xn;
+        }
+        T sQ[GK][R], sD[GK][R], sPin[GK];
+#pragma unroll
+        for (int s = 0; s < GK; ++s) {
           const int t = t0 + s;
           T pinj, qinj;
           injections<T, ANCH>(t, gp.m, C, pinj, qinj);
-          const T Pin = dpp_shr1(pinj, Plast);
-          const T Xn = dpp_shr1(qinj, Q[R - 1]);
-          Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], sQ[s], sD[s]);
-          Xprev = Xn;
+          const T Pin = dpp_shr1(pinj, Pl);
+          const T Xn = dpp_shr1(qinj, Qr[R - 1]);
+          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, cl[t], sQ[s], sD[s],
+                               sub == 0 && s == 0);
+          Xp = Xn;
           sPin[s] = Pin;
         }
-      }
 #pragma unroll
-      for (int s = GK - 1; s >= 0; --s) {
-        if (s < cnt) {
+        for (int s = GK - 1; s >= 0; --s) {
           const int t = t0 + s;
           const uint32_t code = cl[t];
-          const bool post = code == SENT_POST;
+          const bool post = TL && code == SENT_POST;
           const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
           const T Pbin = dpp_shl1(Pseed, Ilast);
           const T Qbin = Eprev;
-          const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
+          const T einj = (TL && ANCH && t - 17 == gp.m) ? tseed : T(0);
           const T En = dpp_shl1(einj, EMp[0]);
           T Qb[R], e[R], sw[R], Iv[R];
 #pragma unroll
@@ -328,7 +343,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const bool mt = rc[r] == code;
-            e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
+            e[r] = mt ? gd[r] : bd[r];
+            if (TL) e[r] = post ? T(0) : e[r];
             sw[r] = mt ? T(-1) : smis;
             Iv[r] = pass[r] ? T(0) : Ih;
             Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
@@ -358,6 +374,140 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           }
           Ilast = Pb;
           Eprev = En;
+        }
+      }
+    };
+    auto load_block = [&](int blk) {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged copy has landed
+#pragma unroll
+      for (int w = 0; w < CKW; ++w) cur[w] = ckl[w * 64 + lane];
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): read before the next copy overwrites it
+      if (blk > 0) stage(blk - 1);
+    };
+    // blocks that reach tmain or beyond test for columns past an end, the
+    // blocks below run without the test (two loops, so the two variants of the
+    // block never merge their registers)
+    const int nmain = tmain / CK_STEPS;
+    int blk = nblk - 1;
+    for (; blk >= nmain; --blk) {
+      load_block(blk);
+      bwd_block(blk * CK_STEPS, std::true_type{});
+    }
+    for (; blk >= 0; --blk) {
+      load_block(blk);
+      bwd_block(blk * CK_STEPS, std::false_type{});
+    }
+  } else {
+    // the anchored terminal of a pair whose haplotype is the unit's longest
+    // enters before the first backward step
+    T Eprev = (ANCH && l == GROUP - 1 && tend - 16 == gp.m) ? tseed : T(0);
+    constexpr bool PREF = sizeof(T) == 4 ? false : R >= 8;
+    T cur[CKW], ckn[PREF ? CKW : 1];
+    if (PREF) {
+      const T* c = ck + (size_t)(nblk - 1) * CKW * 64 + lane;
+#pragma unroll
+      for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
+    }
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+      if (PREF) {
+#pragma unroll
+        for (int w = 0; w < CKW; ++w) cur[w] = ckn[w % (PREF ? CKW : 1)];
+        if (blk > 0) {
+          const T* c = ck + (size_t)(blk - 1) * CKW * 64 + lane;
+#pragma unroll
+          for (int w = 0; w < (PREF ? CKW : 1); ++w) ckn[w] = c[(size_t)w * 64];
+        }
+      } else {
+        const T* c = ck + (size_t)blk * CKW * 64 + lane;
+#pragma unroll
+        for (int w = 0; w < CKW; ++w) cur[w] = c[(size_t)w * 64];
+      }
+      const int tb = blk * CK_STEPS;
+      for (int sub = NSUB - 1; sub >= 0; --sub) {
+        const int t0 = tb + sub * GK;
+        if (t0 > tend) continue;
+        const int cnt = min(GK, tend + 1 - t0);
+        // state before step tb: M = x D^ + M^ and D = 0 make the first step's
+        // D update reproduce D^(tb) exactly
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          M[r] = cur[r];
+          D[r] = 0;
+          Q[r] = cur[R + r];
+        }
+        Plast = cur[2 * R];
+        Xprev = cur[2 * R + 1];
+        for (int t = tb; t < t0; ++t) {  // steps before the register block, not stored
+          T pinj, qinj;
+          injections<T, ANCH>(t, gp.m, C, pinj, qinj);
+          const T Pin = dpp_shr1(pinj, Plast);
+          const T Xn = dpp_shr1(qinj, Q[R - 1]);
+          Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], qu, dd);
+          Xprev = Xn;
+        }
+        T sQ[GK][R], sD[GK][R], sPin[GK];
+#pragma unroll
+        for (int s = 0; s < GK; ++s) {
+          if (s < cnt) {
+            const int t = t0 + s;
+            T pinj, qinj;
+            injections<T, ANCH>(t, gp.m, C, pinj, qinj);
+            const T Pin = dpp_shr1(pinj, Plast);
+            const T Xn = dpp_shr1(qinj, Q[R - 1]);
+            Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, cl[t], sQ[s], sD[s]);
+            Xprev = Xn;
+            sPin[s] = Pin;
+          }
+        }
+#pragma unroll
+        for (int s = GK - 1; s >= 0; --s) {
+          if (s < cnt) {
+            const int t = t0 + s;
+            const uint32_t code = cl[t];
+            const bool post = code == SENT_POST;
+            const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
+            const T Pbin = dpp_shl1(Pseed, Ilast);
+            const T Qbin = Eprev;
+            const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
+            const T En = dpp_shl1(einj, EMp[0]);
+            T Qb[R], e[R], sw[R], Iv[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
+            T Ih = sPin[s];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const bool mt = rc[r] == code;
+              e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
+              sw[r] = mt ? T(-1) : smis;
+              Iv[r] = pass[r] ? T(0) : Ih;
+              Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
+            }
+            T Pb = Pbin;
+#pragma unroll
+            for (int r = R - 1; r >= 0; --r) {
+              const T gr = r == R - 1 ? gam_last : gam;
+              const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
+              const T Mbar = Pb + Qb[r] + Dbr[r];
+              const T EM = e[r] * Mbar;
+              Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
+              const T u = mu[r] * Pb, v = be[r] * Qb[r];
+              if (r == R - 1) {
+                UL = fmaT(u, Iv[r], UL);
+                VL = fmaT(v, Iv[r], VL);
+                Tl = fmaT(Qb[r], sD[s][r], Tl);
+              } else {
+                U = fmaT(u, Iv[r], U);
+                V = fmaT(v, Iv[r], V);
+                Tg = fmaT(Qb[r], sD[s][r], Tg);
+              }
+              Tx = fmaT(Dbr[r], sD[s][r], Tx);
+              Dbr[r] = Dbar;
+              EMp[r] = EM;
+              Pb = u + v;
+            }
+            Ilast = Pb;
+            Eprev = En;
+          }
         }
       }
     }
@@ -435,6 +585,7 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
   const QTab<float>* tab = reinterpret_cast<const QTab<float>*>(lds + 64 * sizeof(double));
   uint16_t* codes =
       reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)));
+  float* ckl = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(codes) + lds_codes_bytes(smax));
   float* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63;
   double acc[2] = {0.0, 0.0};
@@ -450,7 +601,7 @@ __device__ __forceinline__ void grad32_body(const BatchDev& b, const PlanDev& p,
     } else if (decode_unit32(b, p, lo + u, gp, out) != R) {
       continue;
     }
-    unit_grad<float, R, ANCH, GK>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false);
+    unit_grad<float, R, ANCH, GK>(b, p, tab, gt, codes, smax, gp, ck, weights, out, acc, lds_de, false, ckl);
   }
   grad_flush(acc[0], acc[1], lds_de, gpart);
 }
@@ -1020,9 +1171,12 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const double* __restrict__ 
   if (threadIdx.x == 0) grad[i] = part[0];
 }
 
-size_t lds_grad32(int smax) {
-  return 64 * sizeof(double) + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)) +
-         (size_t)PAIRS_PER_UNIT * smax * sizeof(uint16_t);
+// FP32: per-row tables, hap codes, then the staged checkpoint block of a
+// kernel with r rows per lane (r = 0: the largest class)
+size_t lds_grad32(int smax, int r) {
+  const int ckw = r > 0 ? 2 * r + 2 : CKW_MAX;
+  return 64 * sizeof(double) + ((qtab_lds_bytes<float>() + 15) & ~size_t(15)) + lds_codes_bytes(smax) +
+         (size_t)ckw * 64 * sizeof(float);
 }
 size_t lds_grad64(int smax) {
   return 64 * sizeof(double) + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)) +
@@ -1042,8 +1196,8 @@ static hipError_t occ_grad_r(int r, int fp64, int variant, int smax, int* n) {
 #define VD_OCC32(RR)                                                                                        \
   case RR:                                                                                                  \
     return (RR <= GRAD_W2_MAX && variant == 0)                                                              \
-               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32w<ANCH, RR>, 64, lds_grad32(smax)) \
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax));
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32w<ANCH, RR>, 64, lds_grad32(smax, RR)) \
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(n, k_grad32<ANCH, RR>, 64, lds_grad32(smax, RR));
     VD_OCC32(1) VD_OCC32(2) VD_OCC32(3) VD_OCC32(4) VD_OCC32(5) VD_OCC32(6) VD_OCC32(7) VD_OCC32(8)
     VD_OCC32(9) VD_OCC32(10) VD_OCC32(11) VD_OCC32(12) VD_OCC32(13) VD_OCC32(14) VD_OCC32(15) VD_OCC32(16)
 #undef VD_OCC32
@@ -1120,10 +1274,10 @@ hipError_t launch_grad(const BatchDev& b, const PlanDev& p, const int32_t* qE, c
   for (int r = MAX_R32; r >= 1; --r) {
     if (ev_cls) (void)hipEventRecord(ev_cls[r], st);
     if (anchored)
-      launch_grad32_r<true>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
+      launch_grad32_r<true>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax, r), st, b, p, tab32, gt, smax, ck32,
                             s32, weights, out, gpart);
     else
-      launch_grad32_r<false>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax), st, b, p, tab32, gt, smax, ck32,
+      launch_grad32_r<false>(r, variant32[r], dim3(blocks32[r]), lds_grad32(smax, r), st, b, p, tab32, gt, smax, ck32,
                              s32, weights, out, gpart);
   }
   if (ev_cls) (void)hipEventRecord(ev_cls[0], st);

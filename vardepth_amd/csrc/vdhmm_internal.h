@@ -18,6 +18,10 @@ constexpr int MAX_R32 = 16;
 
 constexpr int GROUP = 16;        // lanes per pair (one DPP row)
 constexpr int PAIRS_PER_UNIT = 4;
+// bytes of a unit's hap-code slots in LDS (uint16 per slot, 16-byte multiple)
+__host__ __device__ constexpr size_t lds_codes_bytes(int smax) {
+  return ((size_t)PAIRS_PER_UNIT * smax * 2 + 15) & ~size_t(15);
+}
 constexpr int R64 = 4;           // FP64 rows per lane: 16 lanes x 4 = 64-row tiles
 
 // hap code stream sentinels (hap bytes are 0..255)
@@ -225,7 +229,7 @@ hipError_t launch_ref_pairs(const BatchDev& b, const PairRef* pairs, int count, 
                             float gap, float ext, int anchored, float* ws, double* out, hipStream_t st);
 hipError_t launch_plain_pair(const uint8_t* read, const uint8_t* qual, int n, const uint8_t* hap, int m,
                              const QTab<float>* tab, int anchored, double lnk, double* out, hipStream_t st);
-size_t lds_grad32(int smax);
+size_t lds_grad32(int smax, int r = 0);  // r rows per lane; 0: the largest class
 size_t lds_grad64(int smax);
 size_t grad_ck_stride(int smax, int fp64);
 hipError_t grad_occupancy(int anchored, int r, int fp64, int variant, int smax, int* blocks_per_cu);
