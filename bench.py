@@ -361,7 +361,7 @@ def accuracy_configs(mode: int):
     return out
 
 
-def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
+def grad_bench(regions: int, steps: int, mode: int, tune: bool = True, train_steps: int = 200):
     """Forward and forward+adjoint GCUPS on the HMM-sampled workload (all pairs,
     unit weights: gradient of sum ln L wrt gap, ext and the 64-bin error table).
 
@@ -369,8 +369,9 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     autograd step runs them.  One gradient call also returns ln L per pair
     (its first sweep is the forward, and the FP64 adjoint rescues the pairs
     it flags), so `fused_gcups` / `fused_ratio` count that call alone.
-    `training` is config 4's own run: 200 Adam steps on the true-haplotype
-    loss from a perturbed start, timed, with the recovered parameters."""
+    `training` is config 4's own run: train_steps (200) Adam steps on the
+    true-haplotype loss from a perturbed start, timed, with the recovered
+    parameters (None when train_steps is 0)."""
     import torch
     from vardepth_amd import engine, workloads
     b = workloads.hmm_sampled_regions(seed=41004, regions=regions)
@@ -428,30 +429,32 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True):
     torch.cuda.synchronize()
     truth_w_ms = (time.perf_counter() - t0) / steps * 1e3
     # config 4's training run: the loss -mean ln L over each read's true
-    # haplotype (the true-pair batch), 200 Adam steps from a perturbed start,
-    # each step one forward and one adjoint call through torch.autograd
-    from vardepth_amd import fit as fitmod
-    true_gap, true_ext = 0.01, 0.3
-    err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
-    tp = workloads.true_pairs(b)
-    devt = engine.DeviceBatch(tp)
-    gap0, ext0, err0 = fitmod.perturbed_start(41004, true_gap, true_ext, err_true)
-    fitmod.fit(devt, gap0, ext0, err0, steps=2, report=2)  # warm
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _, hist = fitmod.fit(devt, gap0, ext0, err0, steps=200, lr=0.05, report=200)
-    torch.cuda.synchronize()
-    fit_s = time.perf_counter() - t0
-    used = np.arange(10, 41)
-    lerr = lambda e: float(np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used]))))
-    training = {"workload": "true-haplotype pairs of the same regions (loss -mean ln L)", "pairs": tp.npairs,
-                "cells": tp.cells(), "adam_steps": 200, "seconds": round(fit_s, 3),
-                "ms_per_step": round(fit_s / 200 * 1e3, 3),
-                "fwd_bwd_gcups": round(tp.cells() * 200 / fit_s / 1e9, 2),
-                "gap": [round(hist[0]["gap"], 5), round(hist[-1]["gap"], 5), true_gap],
-                "ext": [round(hist[0]["ext"], 4), round(hist[-1]["ext"], 4), true_ext],
-                "mean_abs_log10_err_q10_40": [round(lerr(hist[0]["err"]), 4), round(lerr(hist[-1]["err"]), 4)],
-                "loss": [round(hist[0]["loss"], 4), round(hist[-1]["loss"], 4)]}
+    # haplotype (the true-pair batch), train_steps Adam steps from a perturbed
+    # start, each step one forward and one adjoint call through torch.autograd
+    training = None
+    if train_steps > 0:
+        from vardepth_amd import fit as fitmod
+        true_gap, true_ext = 0.01, 0.3
+        err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
+        tp = workloads.true_pairs(b)
+        devt = engine.DeviceBatch(tp)
+        gap0, ext0, err0 = fitmod.perturbed_start(41004, true_gap, true_ext, err_true)
+        fitmod.fit(devt, gap0, ext0, err0, steps=2, report=2)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist = fitmod.fit(devt, gap0, ext0, err0, steps=train_steps, lr=0.05, report=train_steps)
+        torch.cuda.synchronize()
+        fit_s = time.perf_counter() - t0
+        used = np.arange(10, 41)
+        lerr = lambda e: float(np.mean(np.abs(np.log10(np.asarray(e)[used]) - np.log10(err_true[used]))))
+        training = {"workload": "true-haplotype pairs of the same regions (loss -mean ln L)", "pairs": tp.npairs,
+                    "cells": tp.cells(), "adam_steps": train_steps, "seconds": round(fit_s, 3),
+                    "ms_per_step": round(fit_s / train_steps * 1e3, 3),
+                    "fwd_bwd_gcups": round(tp.cells() * train_steps / fit_s / 1e9, 2),
+                    "gap": [round(hist[0]["gap"], 5), round(hist[-1]["gap"], 5), true_gap],
+                    "ext": [round(hist[0]["ext"], 4), round(hist[-1]["ext"], 4), true_ext],
+                    "mean_abs_log10_err_q10_40": [round(lerr(hist[0]["err"]), 4), round(lerr(hist[-1]["err"]), 4)],
+                    "loss": [round(hist[0]["loss"], 4), round(hist[-1]["loss"], 4)]}
     fwd_g = cells / fwd_ms / 1e6
     fb_g = cells / (fwd_ms + grad_ms) / 1e6
     fused_g = cells / grad_ms / 1e6
@@ -673,6 +676,8 @@ def main():
                     help="cells of the reference-GPU-design baseline sample (0 = skip)")
     ap.add_argument("--grad-regions", type=int, default=1024, help="regions of the fwd+bwd workload (0 = skip)")
     ap.add_argument("--grad-steps", type=int, default=5)
+    ap.add_argument("--train-steps", type=int, default=200,
+                    help="Adam steps of the gradient leg's training run (0 = skip)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the single-region and precision-stress reports")
     ap.add_argument("--no-scalar", action="store_true", help="skip the scalar drop-in latency leg")
     ap.add_argument("--geno-mb", type=float, default=60.0,
@@ -703,8 +708,8 @@ def main():
         return
 
     if args.no_forward:
-        print(json.dumps({"fwd_bwd": grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)}),
-              flush=True)
+        fb = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune, args.train_steps)
+        print(json.dumps({"fwd_bwd": fb}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -788,7 +793,7 @@ def main():
 
     fwd_bwd = None
     if args.grad_regions > 0:
-        fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune)
+        fwd_bwd = grad_bench(args.grad_regions, args.grad_steps, args.mode, not args.no_tune, args.train_steps)
         print("fwd+bwd: %s" % json.dumps(fwd_bwd), file=sys.stderr, flush=True)
 
     scalar = None
