@@ -114,6 +114,9 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
                                           const GroupPair gp, T* __restrict__ ck, const double* __restrict__ weights,
                                           double* __restrict__ out, double* acc, double* lds_de, bool rescue,
                                           T* ckl = nullptr) {
+  // only the written fmas fuse: every instance and variant rounds alike, so
+  // every launch choice gives the same gradients
+#pragma clang fp contract(off)
   constexpr int CKW = 2 * R + 2;
   const int lane = threadIdx.x & 63, l = lane & 15;
   const int tend = gp.m_max + 16;
@@ -370,7 +373,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
             Tx = fmaT(Dbr[r], sD[s][r], Tx);
             Dbr[r] = Dbar;
             EMp[r] = EM;
-            Pb = u + v;
+            Pb = fmaT(be[r], Qb[r], u);
           }
           Ilast = Pb;
           Eprev = En;
@@ -503,7 +506,7 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
               Tx = fmaT(Dbr[r], sD[s][r], Tx);
               Dbr[r] = Dbar;
               EMp[r] = EM;
-              Pb = u + v;
+              Pb = fmaT(be[r], Qb[r], u);
             }
             Ilast = Pb;
             Eprev = En;
@@ -897,7 +900,7 @@ __device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t*
             Tx = fmaT(Dbr[r], sD[s][r], Tx);
             Dbr[r] = Dbar;
             EMp[r] = EM;
-            Pb = u + v;
+            Pb = fmaT(be[r], Qb[r], u);
           }
           Ilast = Pb;
           Eprev = En;
