@@ -373,7 +373,8 @@ int vd_forward_refdesign(const vd_batch *batch, const vd_params *params, int mod
 
 /* Longest haplotype (bases) a batch may hold: the forward (gradients = 0)
  * and gradient (gradients = 1) kernels keep each work unit's haplotype codes
- * in one workgroup's LDS (160 KiB on gfx950, about 20,000 bases).  Calls with
+ * in one workgroup's LDS (160 KiB on gfx950: 20,104 bases for the forward,
+ * 19,112 for gradients, whose workgroups also stage a checkpoint block).  Calls with
  * a longer haplotype fail with vd_last_error() set before touching the device.
  * No counterpart in the reference (its CPU kernels have no such bound). */
 int vd_max_hap_len(int gradients);
