@@ -696,12 +696,12 @@ def main():
     import torch
     from vardepth_amd import engine
 
+    if args.copy_calib and (args.only_genotype or args.no_forward):
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     if args.only_genotype:
-        if args.copy_calib:
-            src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
-            torch.empty_like(src).copy_(src)
-            torch.cuda.synchronize()
-            del src
         print(json.dumps({"genotype": genotype_bench(args.geno_mb, [float(d) for d in args.geno_depths.split(",")],
                                                      args.mode, max(1, min(args.steps, 3)), args.geno_cpu_seconds,
                                                      min(available_cpus(), CPU_THREAD_CAP), not args.no_tune)}), flush=True)
