@@ -365,7 +365,7 @@ int plan_buffers(const vd_batch* db, int64_t npairs) {
   HIPCHK(g.read_ln32.ensure(nreads * 8));
   HIPCHK(g.class_off.ensure(nreg * (vd::NCLS + 1) * 4));
   HIPCHK(g.read_perm.ensure(nreads * 8));
-  HIPCHK(g.unit_cnt.ensure((nb + 1) * 8));
+  HIPCHK(g.unit_cnt.ensure(vd::unit_cnt_len(nb) * 8));
   // a unit map, or with packed pairs (the gradients) a bucket per FP32 pair
   HIPCHK(g.unit_bucket.ensure(std::max<int64_t>(max_units, npairs + 1) * 4));
   HIPCHK(g.f64_list.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
@@ -555,7 +555,7 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   const size_t o_qE = slice(64 * sizeof(int32_t)), o_t32 = slice(sizeof(vd::QTab<float>)),
                o_t64 = slice(sizeof(vd::QTab<double>)), o_cls = slice(nreads), o_exp = slice(nreads * 4),
                o_ln32 = slice(nreads * 8),
-               o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8), o_ucnt = slice((nb + 1) * 8),
+               o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8), o_ucnt = slice(vd::unit_cnt_len(nb) * 8),
                o_ubkt = slice(max_units * 4), o_f64 = slice(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)),
                o_ctr = slice(vd::CTR_COUNT * 8);
   const size_t ws_block = (size_t)4 * vd::PAIRS_PER_UNIT * smax * sizeof(double);

@@ -678,6 +678,11 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
                                                size_t ck_stride, const PairRef* __restrict__ bins, int64_t bin_cap,
                                                const double* __restrict__ weights, double* __restrict__ out,
                                                double* __restrict__ gpart) {
+  constexpr int BIN = R / 2 - 1;
+  unsigned long long count = p.ctr[CTR_BIN64 + BIN];
+  // an empty bin adds nothing: the partial rows were zeroed at the call's start
+  if (count == 0) return;
+  if ((int64_t)count > bin_cap) count = (unsigned long long)bin_cap;
   extern __shared__ __align__(16) unsigned char lds[];
   grad_lds_init<double>(gtab, qtab_lds_bytes<double>(), lds);
   double* lds_de = reinterpret_cast<double*>(lds);
@@ -686,10 +691,7 @@ __global__ __launch_bounds__(64) void k_grad64(BatchDev b, PlanDev p, const QTab
       reinterpret_cast<uint16_t*>(lds + 64 * sizeof(double) + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
   double* ck = ckall + (size_t)blockIdx.x * ck_stride;
   const int lane = threadIdx.x & 63, g = lane >> 4;
-  constexpr int BIN = R / 2 - 1;
   double acc[2] = {0.0, 0.0};
-  unsigned long long count = p.ctr[CTR_BIN64 + BIN];
-  if ((int64_t)count > bin_cap) count = (unsigned long long)bin_cap;
   const PairRef* list = bins + (size_t)BIN * bin_cap;
   const unsigned long long units = (count + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT;
   for (;;) {
@@ -926,6 +928,8 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
                                                 const double* __restrict__ weights, double* __restrict__ out,
                                                 double* __restrict__ gpart) {
   constexpr int R = R64;
+  // no long read: nothing to add (the partial rows were zeroed at the call's start)
+  if (p.ctr[CTR_BIN64T] == 0) return;
   extern __shared__ __align__(16) unsigned char lds[];
   grad_lds_init<double>(gtab, qtab_lds_bytes<double>(), lds);
   double* lds_de = reinterpret_cast<double*>(lds);

@@ -18,6 +18,10 @@ constexpr int MAX_R32 = 16;
 
 constexpr int GROUP = 16;        // lanes per pair (one DPP row)
 constexpr int PAIRS_PER_UNIT = 4;
+// unit_cnt: the nb + 1 bucket counts (scanned in place), then the unit scan's
+// per-tile sums, one per SCAN_TILE buckets
+constexpr int SCAN_TILE = 4096;
+__host__ __device__ constexpr int64_t unit_cnt_len(int64_t nb) { return nb + 1 + (nb + SCAN_TILE - 1) / SCAN_TILE; }
 // bytes of a unit's hap-code slots in LDS (uint16 per slot, 16-byte multiple)
 __host__ __device__ constexpr size_t lds_codes_bytes(int smax) {
   return ((size_t)PAIRS_PER_UNIT * smax * 2 + 15) & ~size_t(15);
@@ -106,7 +110,7 @@ struct PlanDev {
   double* read_ln32;        // [nreads]  sum of ln rho_q over the read (FP32 normalisers)
   int32_t* class_off;       // [nregions * (NCLS+1)] per-region class offsets
   int64_t* read_perm;       // [nreads]  reads grouped by class inside each region
-  int64_t* unit_cnt;        // [nregions * NCLS + 1] FP32 units per bucket, scanned in place
+  int64_t* unit_cnt;        // [unit_cnt_len(nregions * NCLS)] FP32 units per bucket, scanned in place
   int32_t* unit_bucket;     // [total units] bucket of every unit
   PairRef* f64_list;        // [capacity] FP64 work list
   unsigned long long* ctr;  // counters, see CTR_*

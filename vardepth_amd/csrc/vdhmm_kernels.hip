@@ -139,32 +139,82 @@ __global__ void k_bucket(BatchDev b, PlanDev p, double* __restrict__ out) {
   }
 }
 
-// Single-block exclusive scan of unit_cnt[0..n) in place; unit_cnt[n] and
-// ctr[CTR_NUNITS] receive the total.
-__global__ void k_scan_units(int64_t* __restrict__ a, int64_t n, unsigned long long* ctr) {
-  __shared__ int64_t part[1024];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int64_t chunk = (n + nt - 1) / nt;
-  const int64_t lo = tid * chunk, hi = lo + chunk < n ? lo + chunk : n;
-  int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += a[i];
-  part[tid] = s;
+// Exclusive scan of unit_cnt[0..n) in place, in three passes: the sum of
+// every SCAN_TILE-entry tile, a scan of those sums (one block), each tile
+// scanned from its offset.  unit_cnt[n] and ctr[CTR_NUNITS] receive the
+// total; the tile sums sit after unit_cnt[n].
+constexpr int SCAN_THREADS = 256, SCAN_PER_THREAD = SCAN_TILE / SCAN_THREADS;
+
+// Exclusive scan of one value per thread over a block of NT threads (sh: NT
+// entries of LDS); `total` gets the block's sum.  Every thread must call it.
+template <int NT>
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t* sh, int64_t& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
   __syncthreads();
-  for (int d = 1; d < nt; d <<= 1) {
-    int64_t v = tid >= d ? part[tid - d] : 0;
+  for (int d = 1; d < NT; d <<= 1) {
+    const int64_t x = t >= d ? sh[t - d] : 0;
     __syncthreads();
-    part[tid] += v;
+    sh[t] += x;
     __syncthreads();
   }
-  int64_t run = tid ? part[tid - 1] : 0;
-  for (int64_t i = lo; i < hi; ++i) {
-    int64_t v = a[i];
-    a[i] = run;
-    run += v;
+  const int64_t incl = sh[t];
+  total = sh[NT - 1];
+  __syncthreads();  // sh free for the caller's next scan
+  return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile_sums(const int64_t* __restrict__ a, int64_t n,
+                                                                 int64_t* __restrict__ tsum) {
+  __shared__ int64_t sh[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t s = 0;
+  for (int j = threadIdx.x; j < SCAN_TILE; j += SCAN_THREADS) {
+    const int64_t i = base + j;
+    if (i < n) s += a[i];
   }
-  if (tid == nt - 1) {
-    a[n] = part[nt - 1];
-    ctr[CTR_NUNITS] = (unsigned long long)part[nt - 1];
+  int64_t tot;
+  block_excl_scan<SCAN_THREADS>(s, sh, tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(int64_t* __restrict__ tsum, int64_t nt, int64_t* __restrict__ a,
+                                                     int64_t n, unsigned long long* ctr) {
+  __shared__ int64_t sh[1024];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nt; c0 += 1024) {
+    const int64_t i = c0 + threadIdx.x;
+    const int64_t v = i < nt ? tsum[i] : 0;
+    int64_t tot;
+    const int64_t ex = block_excl_scan<1024>(v, sh, tot);
+    if (i < nt) tsum[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    a[n] = carry;
+    ctr[CTR_NUNITS] = (unsigned long long)carry;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int64_t* __restrict__ a, int64_t n,
+                                                             const int64_t* __restrict__ tsum) {
+  __shared__ int64_t sh[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER_THREAD;
+  int64_t v[SCAN_PER_THREAD];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? a[i] : 0;
+    s += v[k];
+  }
+  int64_t tot;
+  int64_t run = tsum[blockIdx.x] + block_excl_scan<SCAN_THREADS>(s, sh, tot);
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+    const int64_t i = base + k;
+    if (i < n) a[i] = run;
+    run += v[k];
   }
 }
 
@@ -1207,6 +1257,7 @@ __global__ __launch_bounds__(256) void k_f64_scatter(BatchDev b, PlanDev p, int 
 template <bool ANCH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_fwd64(BatchDev b, PlanDev p, const QTab<double>* __restrict__ gtab,
                                               int smax, double* __restrict__ wsall, double* __restrict__ out) {
+  if (p.ctr[CTR_F64] == 0) return;  // nothing rescued in this pass
   extern __shared__ __align__(16) unsigned char lds[];
   QTab<double>* tab = reinterpret_cast<QTab<double>*>(lds);
   uint16_t* codes = reinterpret_cast<uint16_t*>(lds + ((qtab_lds_bytes<double>() + 15) & ~size_t(15)));
@@ -1414,7 +1465,11 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const Q
     hipLaunchKernelGGL(k_bucket, dim3(g), dim3(256), 0, st, b, p, out);
     if (p.trie) hipLaunchKernelGGL(k_hap_trie, dim3(g), dim3(64), 0, st, b, p);
   }
-  hipLaunchKernelGGL(k_scan_units, dim3(1), dim3(1024), 0, st, p.unit_cnt, nb, p.ctr);
+  const int64_t nt = (nb + SCAN_TILE - 1) / SCAN_TILE;
+  int64_t* tsum = p.unit_cnt + nb + 1;
+  if (nt > 0) hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nt), dim3(SCAN_THREADS), 0, st, p.unit_cnt, nb, tsum);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, tsum, nt, p.unit_cnt, nb, p.ctr);
+  if (nt > 0) hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nt), dim3(SCAN_THREADS), 0, st, p.unit_cnt, nb, tsum);
   hipLaunchKernelGGL(k_unit_map, dim3(2048), dim3(256), 0, st, (const int64_t*)p.unit_cnt, nb,
                      (const unsigned long long*)p.ctr, p.unit_bucket);
 }
