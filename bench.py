@@ -438,6 +438,10 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True, train_ste
         err_true = np.power(10.0, -np.minimum(np.arange(64), 60) / 10.0)
         tp = workloads.true_pairs(b)
         devt = engine.DeviceBatch(tp)
+        tune_t = None
+        if tune:  # the true-pair batch's own launch choices (its regions hold few pairs)
+            from vardepth_amd import autotune
+            tune_t = {"forward": autotune.tune_forward(devt, mode=mode), "adjoint": autotune.tune_grad(devt, mode=mode)}
         gap0, ext0, err0 = fitmod.perturbed_start(41004, true_gap, true_ext, err_true)
         fitmod.fit(devt, gap0, ext0, err0, steps=2, report=2)  # warm
         torch.cuda.synchronize()
@@ -454,7 +458,7 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True, train_ste
                     "gap": [round(hist[0]["gap"], 5), round(hist[-1]["gap"], 5), true_gap],
                     "ext": [round(hist[0]["ext"], 4), round(hist[-1]["ext"], 4), true_ext],
                     "mean_abs_log10_err_q10_40": [round(lerr(hist[0]["err"]), 4), round(lerr(hist[-1]["err"]), 4)],
-                    "loss": [round(hist[0]["loss"], 4), round(hist[-1]["loss"], 4)]}
+                    "loss": [round(hist[0]["loss"], 4), round(hist[-1]["loss"], 4)], "autotune": tune_t}
     fwd_g = cells / fwd_ms / 1e6
     fb_g = cells / (fwd_ms + grad_ms) / 1e6
     fused_g = cells / grad_ms / 1e6

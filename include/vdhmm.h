@@ -223,6 +223,10 @@ typedef struct vd_tuning {
   int fwd_route64[17];  /* 1: with precision VD_PREC_MIXED, the reads of this class skip
                            the FP32 pass and go straight to the FP64 rescue (for
                            batches where FP32 would flag most of them) */
+  int fwd_pack;         /* 1: every class runs pair units whose 4 pairs may come from
+                           different regions (no prefix sharing): for batches of
+                           regions with few pairs, where region-bound units would
+                           leave most lanes idle; 0 (default): fwd_share decides */
 } vd_tuning;
 
 int vd_tuning_get(vd_tuning *t);

@@ -306,6 +306,36 @@ def test_prefix_sharing_matches_full_sweeps_bit_for_bit(mode):
         L.vd_suffix_sharing(prev_sfx)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_packed_pair_units_match_region_units_bit_for_bit(mode):
+    """vd_tuning.fwd_pack: pair units whose 4 pairs come from different regions
+    give the same likelihoods as region-bound pair units, bit for bit (FP64
+    rescued pairs included), on uneven regions, one-pair regions (the
+    true-haplotype batch), the precision-stress batch and the edge cases."""
+    from vardepth_amd import autotune
+    L = _lib.lib()
+    prev = L.vd_prefix_sharing(-1)
+    base = autotune.get()
+    hmm = workloads.hmm_sampled_regions(seed=41004, regions=12, reads=30, haps=6)
+    cases = _sharing_cases() + [workloads.true_pairs(hmm), workloads.precision_stress(seed=41003, regions=8)]
+    try:
+        L.vd_prefix_sharing(0)
+        for b in cases:
+            t = autotune.get()
+            t.fwd_pack = 0
+            autotune.set_tuning(t)
+            ref, st0 = engine.forward(b, mode=mode, stats=True)
+            t.fwd_pack = 1
+            autotune.set_tuning(t)
+            packed, st1 = engine.forward(b, mode=mode, stats=True)
+            assert st1["flagged"] == st0["flagged"]
+            assert st1["fp32_units"] <= st0["fp32_units"]
+            np.testing.assert_array_equal(packed, ref)
+    finally:
+        autotune.set_tuning(base)
+        L.vd_prefix_sharing(prev)
+
+
 def _suffix_cases():
     rng = np.random.default_rng(91)
     seq = lambda n: "".join(rng.choice(list("ACGT"), n))

@@ -63,8 +63,8 @@ class VdTuning(ctypes.Structure):
     _fields_ = [("fwd_unroll", ctypes.c_int * 17), ("fwd_blocks_per_cu", ctypes.c_int * 17),
                 ("grad_variant", ctypes.c_int * 17), ("grad_blocks_per_cu", ctypes.c_int * 17),
                 ("fwd_share", ctypes.c_int * 17), ("share_chunk", ctypes.c_int), ("class_overlap", ctypes.c_int),
-                ("fwd_route64", ctypes.c_int * 17)]
-    SCALARS = ("share_chunk", "class_overlap")
+                ("fwd_route64", ctypes.c_int * 17), ("fwd_pack", ctypes.c_int)]
+    SCALARS = ("share_chunk", "class_overlap", "fwd_pack")
 
     def as_dict(self):
         return {name: (int(getattr(self, name)) if name in self.SCALARS else list(getattr(self, name)))

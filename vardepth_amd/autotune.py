@@ -15,6 +15,10 @@ passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
   kernel's tail, but where one class holds most of the work they share the
   CUs with it for its whole run, and their time varies more: they are kept
   only when more than 2% faster;
+* forward: then whether every class runs pair units packed across regions
+  (``vd_tuning.fwd_pack``, no prefix sharing; kept when more than 2% faster
+  end to end): for batches of regions with one or a few pairs, where
+  region-bound units leave most of a wave's lanes idle;
 * forward, precision "mixed": when the FP32 pass flags at least a quarter of
   the pairs, whether the active classes go straight to the FP64 rescue
   (``vd_tuning.fwd_route64``; kept when more than 2% faster, and the routed
@@ -115,6 +119,9 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
     def run():
         dev_batch.forward(mode=mode, precision=precision, stats=True)
 
+    t0 = get()
+    t0.fwd_pack = 0  # the per-class sweep times region-bound units
+    set_tuning(t0)
     run()  # warm: buffers and tables
     lib = _lib.lib()
     pair_ms = {}
@@ -159,9 +166,29 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
               "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
               "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
               "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+    report["pack"] = _tune_pack(dev_batch, mode, precision, reps)
     if precision == "mixed":
         report["route64"] = _tune_route(dev_batch, mode, active, reps)
     return report
+
+
+def _tune_pack(dev_batch, mode, precision, reps):
+    """Time the whole forward with the tuned choices and with every class on
+    pair units packed across regions (vd_tuning.fwd_pack); keep the packing
+    when it is more than 2% faster."""
+    def total():
+        return min(dev_batch.forward(mode=mode, precision=precision, stats=True)["device_ms"]
+                   for _ in range(max(reps, 2)))
+    base = get()
+    base_ms = total()
+    t = _lib.VdTuning.from_buffer_copy(base)
+    t.fwd_pack = 1
+    set_tuning(t)
+    packed_ms = total()
+    keep = packed_ms < 0.98 * base_ms
+    if not keep:
+        set_tuning(base)
+    return {"fwd_pack": int(keep), "tuned_ms": round(float(base_ms), 3), "packed_ms": round(float(packed_ms), 3)}
 
 
 def _tune_route(dev_batch, mode, active, reps, min_flagged=0.25):
@@ -223,6 +250,7 @@ def defaults() -> _lib.VdTuning:
         t.fwd_route64[r] = 0
     t.share_chunk = CHUNK_CANDIDATES[0]
     t.class_overlap = 1
+    t.fwd_pack = 0
     return t
 
 

@@ -344,6 +344,7 @@ void tune_defaults() {
     g.tune.fwd_route64[r] = 0;
   }
   g.tune.share_chunk = 4;
+  g.tune.fwd_pack = 0;
   g.tune_init = true;
 }
 
@@ -426,8 +427,10 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   HIPCHK(g.ws64.ensure((size_t)blocks64 * 4 * vd::PAIRS_PER_UNIT * smax * sizeof(double)));
   HIPCHK(g.f64_sorted.ensure(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)));
   HIPCHK(g.f64_bins.ensure(2 * vd::F64_BINS_TOTAL * sizeof(unsigned long long)));
+  // packed pair units (fwd_pack): every class, no prefix sharing
+  const int pack = g.tune.fwd_pack && !force64 ? 1 : 0;
   int trie = 0;  // class mask of the read units with prefix sharing
-  if (g.share && !force64)
+  if (g.share && !force64 && !pack)
     for (int r = 1; r <= vd::MAX_R32; ++r) trie |= g.tune.fwd_share[r] ? (1 << r) : 0;
   int sfx = 0;    // class mask of the read units with the suffix continuation
   int split = 0;  // of those, classes split between the suffix and the prefix-only kernel
@@ -481,6 +484,7 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
   vd::PlanDev p = plan_dev(npairs, 1, trie);
   p.sfx = sfx;
   p.split = split;
+  p.pack_pairs = pack;
   const bool timing = stats && g.class_timing;
   HIPCHK(vd::launch_pipeline(b, p, g.qE.as<int32_t>(), g.tab32.as<vd::QTab<float>>(),
                              g.tab64.as<vd::QTab<double>>(), anchored, route_mask(force64), smax, blocks32, blocks32s,
@@ -509,7 +513,9 @@ int forward_device_locked(const vd_batch* db, const vd_params* params, int mode,
     stats->fwd32_ms = b2;
     stats->fwd64_ms = c3;
     stats->pairs = (uint64_t)npairs;
-    stats->fp32_units = force64 ? 0 : c[vd::CTR_NUNITS];
+    // packed plans count pairs
+    stats->fp32_units = force64 ? 0 : pack ? (c[vd::CTR_NUNITS] + vd::PAIRS_PER_UNIT - 1) / vd::PAIRS_PER_UNIT
+                                           : c[vd::CTR_NUNITS];
     stats->flagged = c[vd::CTR_FLAGGED];
     stats->fp64_pairs = std::min<unsigned long long>(c[vd::CTR_F64], (unsigned long long)std::max<int64_t>(npairs, 1));
     stats->fp64_bad = c[vd::CTR_F64BAD];
@@ -1596,6 +1602,7 @@ int vd_tuning_set(const vd_tuning* t) {
   }
   if (t->share_chunk < 1 || t->share_chunk > 64) return fail("vdhmm: share_chunk must be in 1..64");
   if (t->class_overlap != 0 && t->class_overlap != 1) return fail("vdhmm: class_overlap must be 0 or 1");
+  if (t->fwd_pack != 0 && t->fwd_pack != 1) return fail("vdhmm: fwd_pack must be 0 or 1");
   std::lock_guard<std::mutex> lk(g_mu);
   g.tune = *t;
   g.tune_init = true;

@@ -988,13 +988,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(wpe_p(ANCH, 
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
+  // the class's units [lo, hi), or with packed pairs its pairs [lo, hi) in
+  // units of 4 that may span regions
+  const unsigned long long nunits = p.pack_pairs ? (hi - lo + PAIRS_PER_UNIT - 1) / PAIRS_PER_UNIT : hi - lo;
   for (;;) {
     unsigned long long u = 0;
-    if (lane == 0) u = lo + atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
+    if (lane == 0) u = atomicAdd(&p.ctr[CTR_CLS + R], 1ull);
     u = first_lane(u);
-    if (u >= hi) break;
+    if (u >= nunits) break;
     GroupPair gp;
-    if (decode_unit32(b, p, u, gp, out) != R) continue;
+    if (p.pack_pairs) {
+      const unsigned long long pi = lo + u * PAIRS_PER_UNIT + (lane >> 4);
+      if (decode_pair32(b, p, (int64_t)pi, pi < hi, gp, out) < 0) continue;
+    } else if (decode_unit32(b, p, lo + u, gp, out) != R) {
+      continue;
+    }
     unit32<R, ANCH, U>(b, p, tab, codes, smax, gp, out);
   }
 }
