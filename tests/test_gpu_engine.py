@@ -336,6 +336,45 @@ def test_packed_pair_units_match_region_units_bit_for_bit(mode):
         L.vd_prefix_sharing(prev)
 
 
+def test_many_regions_scan_in_several_chunks():
+    """300,000 one-pair regions (5.4 M buckets, more tiles than one pass of the
+    tile-sum scan covers): every pair lands in its own output slot with the
+    oracle's value, with region-bound and with packed units."""
+    from vardepth_amd import autotune
+    rng = np.random.default_rng(300)
+    n = 300_000
+    rl = rng.integers(1, 41, n)
+    hl = rng.integers(1, 61, n)
+    bases = np.frombuffer(b"ACGT", np.uint8)
+    reads = bases[rng.integers(0, 4, int(rl.sum()))]
+    haps = bases[rng.integers(0, 4, int(hl.sum()))]
+    ro = np.concatenate([[0], np.cumsum(rl)])
+    ho = np.concatenate([[0], np.cumsum(hl)])
+    b = workloads.RegionBatch(read_bytes=reads.copy(), qual_bytes=rng.integers(5, 41, int(rl.sum())).astype(np.uint8),
+                              read_off=ro.astype(np.int64), hap_bytes=haps.copy(), hap_off=ho.astype(np.int64),
+                              region_reads=np.arange(n + 1, dtype=np.int64),
+                              region_haps=np.arange(n + 1, dtype=np.int64), gap=0.001, ext=0.1)
+    assert b.nregions * 18 > 1024 * 4096
+    sample = rng.choice(n, 400, replace=False)
+    expect = np.array([oracle.dense(b.read_bytes[ro[i]:ro[i + 1]].tobytes().decode(),
+                                    b.qual_bytes[ro[i]:ro[i + 1]].tobytes(),
+                                    b.hap_bytes[ho[i]:ho[i + 1]].tobytes().decode(), 0, b.gap, b.ext)
+                       for i in sample])
+    base = autotune.get()
+    try:
+        outs = []
+        for pack in (0, 1):
+            t = autotune.get()
+            t.fwd_pack = pack
+            autotune.set_tuning(t)
+            out = engine.forward(b, mode=0)
+            close(out[sample], expect, TOL_MIXED)
+            outs.append(out)
+        np.testing.assert_array_equal(outs[0], outs[1])
+    finally:
+        autotune.set_tuning(base)
+
+
 def _suffix_cases():
     rng = np.random.default_rng(91)
     seq = lambda n: "".join(rng.choice(list("ACGT"), n))
