@@ -108,6 +108,70 @@ __device__ __forceinline__ void injections(int t, int m, T C, T& pinj, T& qinj) 
   }
 }
 
+// The adjoint's per-lane sums (see the top of this file).
+template <typename T>
+struct GradSums {
+  T U, V, UL, VL, Tg, Tl, Tx;
+};
+
+// One backward anti-diagonal step: the transposed recurrence over the
+// lane's R rows, bottom to top, from the step's stored diagonal sources (sQ),
+// D^ (sD) and injected P (sPin), with the gradient sums.  The reverse carries
+// come from lane l + 1; lane 15 takes pold into its P carry and qold into the
+// carry that becomes its next step's Qb (the seeds).  TAIL: the step may see
+// columns past a haplotype end (code SENT_POST: emission and emission-sum
+// weight zero).
+template <typename T, int R, bool TAIL>
+__device__ __forceinline__ void bwd_step(uint32_t code, T pold, T qold, const T (&sQ)[R], const T (&sD)[R], T sPin,
+                                         const T (&gd)[R], const T (&bd)[R], const T (&mu)[R], const T (&be)[R],
+                                         const uint32_t (&rc)[R], const bool (&pass)[R], T gam, T gam_last, T xx,
+                                         T (&EMp)[R], T (&Dbr)[R], T (&Er)[R], GradSums<T>& gs, T& Ilast, T& Eprev) {
+#pragma clang fp contract(off)
+  const bool post = TAIL && code == SENT_POST;
+  const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
+  const T Pbin = dpp_shl1(pold, Ilast);
+  const T Qbin = Eprev;
+  const T En = dpp_shl1(qold, EMp[0]);
+  T Qb[R], e[R], sw[R], Iv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
+  T Ih = sPin;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool mt = rc[r] == code;
+    e[r] = mt ? gd[r] : bd[r];
+    if (TAIL) e[r] = post ? T(0) : e[r];
+    sw[r] = mt ? T(-1) : smis;
+    Iv[r] = pass[r] ? T(0) : Ih;
+    Ih = fmaT(mu[r], Ih, e[r] * sQ[r]);
+  }
+  T Pb = Pbin;
+#pragma unroll
+  for (int r = R - 1; r >= 0; --r) {
+    const T gr = r == R - 1 ? gam_last : gam;
+    const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
+    const T Mbar = Pb + Qb[r] + Dbr[r];
+    const T EM = e[r] * Mbar;
+    Er[r] = fmaT(Mbar * sQ[r], sw[r], Er[r]);
+    const T u = mu[r] * Pb, v = be[r] * Qb[r];
+    if (r == R - 1) {
+      gs.UL = fmaT(u, Iv[r], gs.UL);
+      gs.VL = fmaT(v, Iv[r], gs.VL);
+      gs.Tl = fmaT(Qb[r], sD[r], gs.Tl);
+    } else {
+      gs.U = fmaT(u, Iv[r], gs.U);
+      gs.V = fmaT(v, Iv[r], gs.V);
+      gs.Tg = fmaT(Qb[r], sD[r], gs.Tg);
+    }
+    gs.Tx = fmaT(Dbr[r], sD[r], gs.Tx);
+    Dbr[r] = Dbar;
+    EMp[r] = EM;
+    Pb = fmaT(be[r], Qb[r], u);
+  }
+  Ilast = Pb;
+  Eprev = En;
+}
+
 template <typename T, int R, bool ANCH, int GK>
 __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, const QTab<T>* __restrict__ tab,
                                           const GradTab* __restrict__ gt, uint16_t* codes, int smax,
@@ -270,7 +334,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
   T EMp[R], Dbr[R], Er[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) EMp[r] = Dbr[r] = Er[r] = 0;
-  T U = 0, V = 0, UL = 0, VL = 0, Tg = 0, Tl = 0, Tx = 0, Ilast = 0;
+  GradSums<T> gs{0, 0, 0, 0, 0, 0, 0};
+  T Ilast = 0;
   const T Pseed = ANCH ? T(0) : tseed;
   const int nblk = tend / CK_STEPS + 1;
   if constexpr (sizeof(T) == 4) {
@@ -332,51 +397,10 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
 #pragma unroll
         for (int s = GK - 1; s >= 0; --s) {
           const int t = t0 + s;
-          const uint32_t code = cl[t];
-          const bool post = TL && code == SENT_POST;
-          const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
-          const T Pbin = dpp_shl1(Pseed, Ilast);
-          const T Qbin = Eprev;
-          const T einj = (TL && ANCH && t - 17 == gp.m) ? tseed : T(0);
-          const T En = dpp_shl1(einj, EMp[0]);
-          T Qb[R], e[R], sw[R], Iv[R];
-#pragma unroll
-          for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
-          T Ih = sPin[s];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const bool mt = rc[r] == code;
-            e[r] = mt ? gd[r] : bd[r];
-            if (TL) e[r] = post ? T(0) : e[r];
-            sw[r] = mt ? T(-1) : smis;
-            Iv[r] = pass[r] ? T(0) : Ih;
-            Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
-          }
-          T Pb = Pbin;
-#pragma unroll
-          for (int r = R - 1; r >= 0; --r) {
-            const T gr = r == R - 1 ? gam_last : gam;
-            const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
-            const T Mbar = Pb + Qb[r] + Dbr[r];
-            const T EM = e[r] * Mbar;
-            Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
-            const T u = mu[r] * Pb, v = be[r] * Qb[r];
-            if (r == R - 1) {
-              UL = fmaT(u, Iv[r], UL);
-              VL = fmaT(v, Iv[r], VL);
-              Tl = fmaT(Qb[r], sD[s][r], Tl);
-            } else {
-              U = fmaT(u, Iv[r], U);
-              V = fmaT(v, Iv[r], V);
-              Tg = fmaT(Qb[r], sD[s][r], Tg);
-            }
-            Tx = fmaT(Dbr[r], sD[s][r], Tx);
-            Dbr[r] = Dbar;
-            EMp[r] = EM;
-            Pb = fmaT(be[r], Qb[r], u);
-          }
-          Ilast = Pb;
-          Eprev = En;
+          // the anchored terminal enters lane 15 at step m + 17 (past the main blocks)
+          const T qold = (TL && ANCH && t - 17 == gp.m) ? tseed : T(0);
+          bwd_step<T, R, TL>(cl[t], Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam, gam_last, xx,
+                             EMp, Dbr, Er, gs, Ilast, Eprev);
         }
       }
     };
@@ -466,50 +490,9 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
         for (int s = GK - 1; s >= 0; --s) {
           if (s < cnt) {
             const int t = t0 + s;
-            const uint32_t code = cl[t];
-            const bool post = code == SENT_POST;
-            const T smis = post ? T(0) : T(1.0 / 3.0);  // emission-sum weight of a mismatch
-            const T Pbin = dpp_shl1(Pseed, Ilast);
-            const T Qbin = Eprev;
-            const T einj = (ANCH && t - 17 == gp.m) ? tseed : T(0);
-            const T En = dpp_shl1(einj, EMp[0]);
-            T Qb[R], e[R], sw[R], Iv[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
-            T Ih = sPin[s];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const bool mt = rc[r] == code;
-              e[r] = post ? T(0) : (mt ? gd[r] : bd[r]);
-              sw[r] = mt ? T(-1) : smis;
-              Iv[r] = pass[r] ? T(0) : Ih;
-              Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
-            }
-            T Pb = Pbin;
-#pragma unroll
-            for (int r = R - 1; r >= 0; --r) {
-              const T gr = r == R - 1 ? gam_last : gam;
-              const T Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
-              const T Mbar = Pb + Qb[r] + Dbr[r];
-              const T EM = e[r] * Mbar;
-              Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
-              const T u = mu[r] * Pb, v = be[r] * Qb[r];
-              if (r == R - 1) {
-                UL = fmaT(u, Iv[r], UL);
-                VL = fmaT(v, Iv[r], VL);
-                Tl = fmaT(Qb[r], sD[s][r], Tl);
-              } else {
-                U = fmaT(u, Iv[r], U);
-                V = fmaT(v, Iv[r], V);
-                Tg = fmaT(Qb[r], sD[s][r], Tg);
-              }
-              Tx = fmaT(Dbr[r], sD[s][r], Tx);
-              Dbr[r] = Dbar;
-              EMp[r] = EM;
-              Pb = fmaT(be[r], Qb[r], u);
-            }
-            Ilast = Pb;
-            Eprev = En;
+            const T qold = (ANCH && t - 17 == gp.m) ? tseed : T(0);
+            bwd_step<T, R, true>(cl[t], Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam, gam_last,
+                                 xx, EMp, Dbr, Er, gs, Ilast, Eprev);
           }
         }
       }
@@ -530,14 +513,14 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
         atomicAdd(&lds_de[q], (double)Er[r] * (sizeof(T) == 4 ? gt->dgood_de32[q] : gt->dgood_de[q]) * -1.0);
       }
     }
-    const double u = (double)U + (last_lane ? 0.0 : (double)UL);
-    const double v = (double)V + (last_lane ? 0.0 : (double)VL);
-    const double w = last_lane ? (double)UL + (double)VL : 0.0;
-    const double tg = (double)Tg + (last_lane ? 0.0 : (double)Tl);
-    const double tl = last_lane ? (double)Tl : 0.0;
+    const double u = (double)gs.U + (last_lane ? 0.0 : (double)gs.UL);
+    const double v = (double)gs.V + (last_lane ? 0.0 : (double)gs.VL);
+    const double w = last_lane ? (double)gs.UL + (double)gs.VL : 0.0;
+    const double tg = (double)gs.Tg + (last_lane ? 0.0 : (double)gs.Tl);
+    const double tl = last_lane ? (double)gs.Tl : 0.0;
     const double mocc = last_lane ? (double)gp.n * wpair : 0.0;  // n w, once per pair
     acc[0] += -2.0 / a * (mocc - (u + v + w)) + v / (g * a) + w / g + tg * gt->dgam_dg + (ANCH ? tl : 0.0);
-    acc[1] += u / x - v / (1.0 - x) + tg * gt->dgam_dx + (double)Tx;
+    acc[1] += u / x - v / (1.0 - x) + tg * gt->dgam_dx + (double)gs.Tx;
   }
   __syncthreads();  // codes reused by the next unit
 }
@@ -763,8 +746,7 @@ __device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t*
                                             double sc1, double sc2, const double* __restrict__ bP,
                                             const double* __restrict__ bQ, double* __restrict__ tP,
                                             double* __restrict__ tQ, double tseed, double* __restrict__ ck,
-                                            double (&Er)[R64], double& U, double& V, double& UL, double& VL,
-                                            double& Tg, double& Tl, double& Tx) {
+                                            double (&Er)[R64], GradSums<double>& gs) {
   constexpr int R = R64, CKW = 2 * R + 2, CK_STEPS = GKT * NSUB;
   const int lane = threadIdx.x & 63, l = lane & 15;
   // this tile's injection at step t (lane 0 takes it through DPP's old operand)
@@ -863,54 +845,13 @@ __device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t*
       for (int s = GKT - 1; s >= 0; --s) {
         if (s < cnt) {
           const int t = t0 + s;
-          const uint32_t code = cl[t];
-          const bool post = code == SENT_POST;
-          const double smis = post ? 0.0 : 1.0 / 3.0;
-          const double Pbin = dpp_shl1(seed_p(t), Ilast);
-          const double Qbin = Eprev;
-          const double En = dpp_shl1(seed_q(t), EMp[0]);
-          double Qb[R], e[R], sw[R], Iv[R];
-#pragma unroll
-          for (int r = 0; r < R; ++r) Qb[r] = r == R - 1 ? Qbin : EMp[r + 1];
-          double Ih = sPin[s];
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const bool mt = rc[r] == code;
-            e[r] = post ? 0.0 : (mt ? gd[r] : bd[r]);
-            sw[r] = mt ? -1.0 : smis;
-            Iv[r] = pass[r] ? 0.0 : Ih;
-            Ih = fmaT(mu[r], Ih, e[r] * sQ[s][r]);
-          }
-          double Pb = Pbin;
-#pragma unroll
-          for (int r = R - 1; r >= 0; --r) {
-            const double gr = r == R - 1 ? gam_last : gam;
-            const double Dbar = fmaT(xx, Dbr[r], gr * Qb[r]);
-            const double Mbar = Pb + Qb[r] + Dbr[r];
-            const double EM = e[r] * Mbar;
-            Er[r] = fmaT(Mbar * sQ[s][r], sw[r], Er[r]);
-            const double u = mu[r] * Pb, v = be[r] * Qb[r];
-            if (r == R - 1) {
-              UL = fmaT(u, Iv[r], UL);
-              VL = fmaT(v, Iv[r], VL);
-              Tl = fmaT(Qb[r], sD[s][r], Tl);
-            } else {
-              U = fmaT(u, Iv[r], U);
-              V = fmaT(v, Iv[r], V);
-              Tg = fmaT(Qb[r], sD[s][r], Tg);
-            }
-            Tx = fmaT(Dbr[r], sD[s][r], Tx);
-            Dbr[r] = Dbar;
-            EMp[r] = EM;
-            Pb = fmaT(be[r], Qb[r], u);
-          }
-          Ilast = Pb;
-          Eprev = En;
+          bwd_step<double, R, true>(cl[t], seed_p(t), seed_q(t), sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam,
+                                    gam_last, xx, EMp, Dbr, Er, gs, Ilast, Eprev);
           // lane 0: adjoints of this step's injection, Pin (column t - 1) and
           // the diagonal source Qin (column t - 2), times the injection's
           // factors for the tile above
           if (!FIRST && l == 0) {
-            tP[t + 16] = Pb * sc1 * sc2;
+            tP[t + 16] = Ilast * sc1 * sc2;
             tQ[t + 15] = EMp[0] * sc1 * sc2;
           }
         }
@@ -1095,7 +1036,7 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
     }
     __threadfence();
     __syncthreads();
-    double U = 0.0, V = 0.0, UL = 0.0, VL = 0.0, Tg = 0.0, Tl = 0.0, Tx = 0.0;
+    GradSums<double> gs{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double Ufin = 0.0, Vfin = 0.0, Tfin = 0.0;  // the read's last row: lane 15, register R-1 of the last tile
     for (int tile = ntiles - 1; tile >= 0; --tile) {
       const bool first = tile == 0, last = tile == ntiles - 1;
@@ -1114,7 +1055,7 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
       const double ts = flag ? 0.0 : seed;
 #define VD_TG(F, L)                                                                                                 \
   tile_grad64<ANCH, F, L>(gp, cl, tend, gd, bd, mu, be, rc, pass, gam, gam_last, xx, inP, inQ, f1, f2, bPp, bQp, tPp, \
-                          tQp, ts, ck, Er, U, V, UL, VL, Tg, Tl, Tx)
+                          tQp, ts, ck, Er, gs)
       if (first && last) VD_TG(true, true);
       else if (first) VD_TG(true, false);
       else if (last) VD_TG(false, true);
@@ -1122,15 +1063,15 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
 #undef VD_TG
       // in every tile but the last, lane 15's last register row is an interior row
       if (last && l == GROUP - 1) {
-        Ufin = UL;
-        Vfin = VL;
-        Tfin = Tl;
+        Ufin = gs.UL;
+        Vfin = gs.VL;
+        Tfin = gs.Tl;
       } else {
-        U += UL;
-        V += VL;
-        Tg += Tl;
+        gs.U += gs.UL;
+        gs.V += gs.VL;
+        gs.Tg += gs.Tl;
       }
-      UL = VL = Tl = 0.0;
+      gs.UL = gs.VL = gs.Tl = 0.0;
       if (gp.valid && !flag) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -1144,8 +1085,9 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
       const bool last_lane = l == GROUP - 1;
       const double ww = Ufin + Vfin;  // zero off lane 15
       const double mocc = last_lane ? (double)gp.n * wpair : 0.0;
-      acc[0] += -2.0 / a * (mocc - (U + V + ww)) + V / (gg * a) + ww / gg + Tg * gt->dgam_dg + (ANCH ? Tfin : 0.0);
-      acc[1] += U / x - V / (1.0 - x) + Tg * gt->dgam_dx + Tx;
+      acc[0] += -2.0 / a * (mocc - (gs.U + gs.V + ww)) + gs.V / (gg * a) + ww / gg + gs.Tg * gt->dgam_dg +
+                (ANCH ? Tfin : 0.0);
+      acc[1] += gs.U / x - gs.V / (1.0 - x) + gs.Tg * gt->dgam_dx + gs.Tx;
     }
     __syncthreads();  // codes reused by the next unit
   }
