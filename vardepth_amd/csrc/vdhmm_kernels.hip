@@ -40,32 +40,46 @@ namespace vd {
 // planning
 
 // Class and normaliser sums of every read (route64: bit 0 sends every read to
-// the FP64 path, bit c the reads of class c): one wave per read, coalesced
-// loads of the quality bytes, wave reductions of the per-bin FP64 exponents
-// and of the FP32 ln rho (lane partial sums in a fixed order, then a fixed
-// butterfly, so the FP64 sum of a read does not depend on the batch).
-__global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, const QTab<float>* __restrict__ tab32,
-                             int anchored, int route64, int8_t* __restrict__ cls, int32_t* __restrict__ exps,
-                             double* __restrict__ ln32) {
-  const int lane = threadIdx.x & 63;
+// the FP64 path, bit c the reads of class c): one 16-lane group per read, the
+// per-bin tables in LDS, lane partial sums in a fixed order, then a fixed
+// 16-lane butterfly, so the FP64 sum of a read does not depend on the batch.
+template <int XM>
+__device__ __forceinline__ int swz_xor_i(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, (XM << 10) | 0x1f);  // bitmask mode: xor XM within 32 lanes
+}
+
+__global__ __launch_bounds__(256) void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE,
+                                                    const QTab<float>* __restrict__ tab32, int anchored, int route64,
+                                                    int8_t* __restrict__ cls, int32_t* __restrict__ exps,
+                                                    double* __restrict__ ln32) {
+  __shared__ int sE[64];
+  __shared__ double sL[64];
+  for (int i = threadIdx.x; i < 64; i += blockDim.x) {
+    sE[i] = qE[i];
+    sL[i] = tab32->lnrho[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
   const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r = wave0; r < b.nreads; r += nwaves) {
-    const int64_t o0 = b.read_off[r], o1 = b.read_off[r + 1];
+  for (int64_t rb = wave0 * 4; rb < b.nreads; rb += nwaves * 4) {  // uniform over the wave
+    const int64_t r = rb + g;
+    const bool ok = r < b.nreads;
+    const int64_t o0 = ok ? b.read_off[r] : 0, o1 = ok ? b.read_off[r + 1] : 0;
     int e = 0;
-    double l = 0.0;
-    for (int64_t k = o0 + lane; k < o1; k += 64) {
+    double lsum = 0.0;
+    for (int64_t k = o0 + l16; k < o1; k += 16) {
       int q = b.qual_bytes[k];
       q = q < 63 ? q : 63;
-      e += qE[q];
-      l += tab32->lnrho[q];
+      e += sE[q];
+      lsum += sL[q];
     }
-#pragma unroll
-    for (int off = 32; off; off >>= 1) {
-      e += __shfl_xor(e, off);
-      l += __shfl_xor(l, off);
-    }
-    if (lane == 0) {
+    e += swz_xor_i<1>(e);
+    e += swz_xor_i<2>(e);
+    e += swz_xor_i<4>(e);
+    e += swz_xor_i<8>(e);
+    lsum = group_sum(lsum);
+    if (ok && l16 == 0) {
       const int n = (int)(o1 - o0);
       const int np = n + anchored;
       const int c32 = (np + GROUP - 1) / GROUP;
@@ -73,7 +87,7 @@ __global__ void k_prep_reads(BatchDev b, const int32_t* __restrict__ qE, const Q
       const int c = n == 0 ? CLS_EMPTY : (to64 ? CLS_F64 : c32);
       cls[r] = (int8_t)c;
       exps[r] = e;
-      ln32[r] = l;
+      ln32[r] = lsum;
     }
   }
 }
@@ -1463,9 +1477,9 @@ void launch_plan(const BatchDev& b, const PlanDev& p, const int32_t* qE, const Q
   const int64_t nb = b.nregions * NCLS;
   hipLaunchKernelGGL(k_zero_ctr, dim3(1), dim3(64), 0, st, p.ctr, (int)CTR_COUNT);
   if (b.nreads > 0) {
-    int g = (int)((b.nreads + 3) / 4);  // 4 waves of 64 per block, one read per wave
+    int64_t g = (b.nreads + 15) / 16;  // 4 waves of 64 per block, a read per 16-lane group
     g = g > 8192 ? 8192 : g;
-    hipLaunchKernelGGL(k_prep_reads, dim3(g), dim3(256), 0, st, b, qE, tab32, anchored, route64, p.read_cls,
+    hipLaunchKernelGGL(k_prep_reads, dim3((unsigned)g), dim3(256), 0, st, b, qE, tab32, anchored, route64, p.read_cls,
                        p.read_exp, p.read_ln32);
   }
   if (b.nregions > 0) {
