@@ -561,8 +561,8 @@ int forward_noalloc_locked(const vd_batch* db, const vd_params* params, int mode
   const size_t o_qE = slice(64 * sizeof(int32_t)), o_t32 = slice(sizeof(vd::QTab<float>)),
                o_t64 = slice(sizeof(vd::QTab<double>)), o_cls = slice(nreads), o_exp = slice(nreads * 4),
                o_ln32 = slice(nreads * 8),
-               o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8), o_ucnt = slice(vd::unit_cnt_len(nb) * 8),
-               o_ubkt = slice(max_units * 4), o_f64 = slice(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)),
+               o_coff = slice(nreg * (vd::NCLS + 1) * 4), o_perm = slice(nreads * 8),
+               o_ucnt = slice(vd::unit_cnt_len(nb) * 8), o_ubkt = slice(max_units * 4), o_f64 = slice(std::max<int64_t>(npairs, 1) * sizeof(vd::PairRef)),
                o_ctr = slice(vd::CTR_COUNT * 8);
   const size_t ws_block = (size_t)4 * vd::PAIRS_PER_UNIT * smax * sizeof(double);
   if (off + ws_block > cap)

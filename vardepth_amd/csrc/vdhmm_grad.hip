@@ -351,7 +351,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
     auto stage = [&](int blk) {
       const T* c = ck + (size_t)blk * CKW * 64 + lane;
 #pragma unroll
-      for (int w = 0; w < CKW; ++w) __builtin_amdgcn_global_load_lds(c + (size_t)w * 64, (lds_ptr)(ckl + w * 64), 4, 0, 0);
+      for (int w = 0; w < CKW; ++w)
+        __builtin_amdgcn_global_load_lds(c + (size_t)w * 64, (lds_ptr)(ckl + w * 64), 4, 0, 0);
     };
     stage(nblk - 1);
     // one checkpoint block, its NSUB register blocks last to first
