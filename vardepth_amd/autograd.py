@@ -25,10 +25,12 @@ class PairHMMLogLik(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, gap, ext, err, dev_batch: DeviceBatch, mode: int = 0):
-        g, x = float(gap.detach()), float(ext.detach())
-        e = err.detach().to(torch.float64).cpu().numpy()
-        if e.shape != (64,):
+        if err.shape != (64,):
             raise ValueError("err must have 64 quality bins")
+        # the parameter tables are built on the host: one copy back for all 66 values
+        host = torch.cat([gap.detach().reshape(1).to(torch.float64), ext.detach().reshape(1).to(torch.float64),
+                          err.detach().to(torch.float64)]).cpu().numpy()
+        g, x, e = float(host[0]), float(host[1]), host[2:].copy()
         dev_batch.forward(mode=mode, precision="mixed", err=e, gap=g, ext=x)
         out = dev_batch.out[:dev_batch.npairs].clone()
         ctx.dev_batch, ctx.mode, ctx.params = dev_batch, mode, (g, x, e)
@@ -40,7 +42,8 @@ class PairHMMLogLik(torch.autograd.Function):
         g, x, e = ctx.params
         gvec = ctx.dev_batch.grad(gap=g, ext=x, err=e, mode=ctx.mode, weights=grad_out)
         gd, gdev, xd, xdev, ed, edev = ctx.meta
-        dgap = gvec[0].to(dtype=gd, device=gdev)
+        gvec = gvec.to(device=gdev)  # one copy when the parameters live on one device (the host, typically)
+        dgap = gvec[0].to(dtype=gd)
         dext = gvec[1].to(dtype=xd, device=xdev)
         derr = gvec[2:].to(dtype=ed, device=edev)
         return dgap, dext, derr, None, None
