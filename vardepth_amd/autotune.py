@@ -8,7 +8,8 @@ passes, not candidates x classes.  Choices (``vd_tuning`` in include/vdhmm.h):
 * forward: pair units, haplotype prefix sharing, or prefix sharing with the
   shared-suffix continuation; anti-diagonals per main-loop iteration (2 or 4,
   pair units) x persistent grid per CU (the occupancy limit, or 2 or 3 blocks
-  per CU); before the per-class sweep, the haplotypes per read unit under
+  per CU), the per-class picks kept only when the whole forward beats the
+  default choices; before the per-class sweep, the haplotypes per read unit under
   sharing (4, 8 or 16), timed for the whole forward with the class kernels
   on two streams and on one; after it, the stream layout
   (``vd_tuning.class_overlap``) with the final choices. Two streams fill each
@@ -114,8 +115,14 @@ def _apply_grad(t, r, cand):
     t.grad_variant[r], t.grad_blocks_per_cu[r] = cand
 
 
-def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CANDIDATES):
-    """Sweep the forward choices on a DeviceBatch and apply the best per class."""
+def tune_forward(dev_batch, mode=0, precision="mixed", reps=3, candidates=FWD_CANDIDATES):
+    """Sweep the forward choices on a DeviceBatch and apply the best per class.
+
+    Each candidate's class times are the minimum over `reps` passes (with 2,
+    run-to-run noise of a few percent flipped close classes between pair units
+    and prefix sharing and cost up to 3% of the pass).  The per-class picks are
+    then checked as a whole against the default choices, whose kernels overlap
+    on two streams the same way, and kept only when the whole forward is faster."""
     def run():
         dev_batch.forward(mode=mode, precision=precision, stats=True)
 
@@ -152,6 +159,20 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
         _apply_fwd(t, r, c)
     set_tuning(t)
     tuned = sum(times[c][r] for r, c in choice.items())
+    # the picks as a whole against the default choices (whole-forward time)
+    def whole():
+        return min(dev_batch.forward(mode=mode, precision=precision, stats=True)["fwd32_ms"] for _ in range(reps))
+    picked_ms = whole()
+    t_def = _lib.VdTuning.from_buffer_copy(t)
+    for r in range(1, 17):
+        _apply_fwd(t_def, r, candidates[0])
+    set_tuning(t_def)
+    default_whole_ms = whole()
+    if picked_ms <= default_whole_ms:
+        set_tuning(t)
+    else:
+        choice = {r: candidates[0] for r in choice}
+        tuned = sum(times[candidates[0]][r] for r in choice)
     # the stream layout with the final choices: two streams are kept only when
     # they beat one by more than 2%, since their time varies more from run to
     # run (it depends on how the two kernels' blocks interleave)
@@ -165,7 +186,8 @@ def tune_forward(dev_batch, mode=0, precision="mixed", reps=2, candidates=FWD_CA
     report = {"kind": "forward", "choice": {r: list(c) for r, c in choice.items()},
               "share_chunk": int(get().share_chunk), "chunk_ms": {k: round(float(v), 3) for k, v in chunk_ms.items()},
               "class_overlap": int(overlap), "overlap_ms": {k: round(float(v), 3) for k, v in overlap_ms.items()},
-              "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3)}
+              "default_ms": round(float(np.sum(default[active])), 3), "tuned_ms": round(float(tuned), 3),
+              "whole_ms": {"picked": round(float(picked_ms), 3), "default": round(float(default_whole_ms), 3)}}
     report["pack"] = _tune_pack(dev_batch, mode, precision, reps)
     if precision == "mixed":
         report["route64"] = _tune_route(dev_batch, mode, active, reps)
