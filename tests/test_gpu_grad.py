@@ -250,6 +250,33 @@ def test_autograd_function():
     assert abs(float(g_) - 0.01) < 1e-6 and abs(float(x_) - 0.2) < 1e-6
 
 
+def test_autograd_fused_matches_two_calls():
+    """loglik(fused=True): the forward's unit-weight adjoint, scaled, gives the
+    gradient of a uniform loss (-mean ln L) as the forward + weighted adjoint
+    does; a non-uniform incoming gradient falls back to the weighted adjoint."""
+    import torch
+    from vardepth_amd import autograd
+    rng = np.random.default_rng(13)
+    pairs = rand_pairs(rng, 24)
+    dev = engine.DeviceBatch(batch_of(pairs, 0.01, 0.2))
+    err = perturbed_err(3)
+    w = torch.tensor(rng.uniform(0.5, 2.0, len(pairs)), dtype=torch.float64)
+
+    def grads(fused, weighted):
+        gap = torch.tensor(0.01, dtype=torch.float32, requires_grad=True)
+        ext = torch.tensor(0.2, dtype=torch.float32, requires_grad=True)
+        e = torch.tensor(err, dtype=torch.float32, requires_grad=True)
+        ll = autograd.loglik(gap, ext, e, dev, 0, fused=fused)
+        loss = -(w.to(ll.device) * ll).sum() if weighted else -ll.sum() / len(pairs)
+        loss.backward()
+        return ll.detach().cpu().numpy(), np.concatenate([[gap.grad.item(), ext.grad.item()], e.grad.numpy()])
+    for weighted in (False, True):
+        ll0, g0 = grads(False, weighted)
+        ll1, g1 = grads(True, weighted)
+        np.testing.assert_allclose(ll1, ll0, rtol=0, atol=TOL_MIXED)
+        np.testing.assert_allclose(g1, g0, rtol=1e-5, atol=1e-7 * np.max(np.abs(g0)))
+
+
 def test_adam_recovers_parameters():
     """200 Adam steps through the HIP forward/adjoint from a perturbed start move
     gap, continuation and the populated error bins towards the generator's."""

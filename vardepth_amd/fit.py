@@ -33,7 +33,7 @@ def fit(dev_batch, gap0, ext0, err0, steps=200, lr=0.05, mode=0, report=20, log=
     n = dev_batch.npairs
     for step in range(steps + 1):
         opt.zero_grad()
-        ll = params(dev_batch, mode)
+        ll = params(dev_batch, mode, fused=step < steps)  # the loss -mean ln L: one adjoint call per step
         loss = -ll.sum() / n
         if step % report == 0 or step == steps:
             g, x, e = params.values()
