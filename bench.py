@@ -369,7 +369,7 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True, train_ste
     autograd step runs them.  One gradient call also returns ln L per pair
     (its first sweep is the forward, and the FP64 adjoint rescues the pairs
     it flags), so `fused_gcups` / `fused_ratio` count that call alone.
-    `training` is config 4's own run: train_steps (200) Adam steps on the
+    `training` is the differentiation workload's own run: train_steps (200) Adam steps on the
     true-haplotype loss from a perturbed start, timed, with the recovered
     parameters (None when train_steps is 0)."""
     import torch
@@ -428,7 +428,7 @@ def grad_bench(regions: int, steps: int, mode: int, tune: bool = True, train_ste
         dev.grad(mode=mode, weights=wdev)
     torch.cuda.synchronize()
     truth_w_ms = (time.perf_counter() - t0) / steps * 1e3
-    # config 4's training run: the loss -mean ln L over each read's true
+    # the training run: the loss -mean ln L over each read's true
     # haplotype (the true-pair batch), train_steps Adam steps from a perturbed
     # start, each step one forward and one adjoint call through torch.autograd
     training = None
