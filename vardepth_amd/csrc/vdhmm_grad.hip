@@ -166,7 +166,7 @@ __device__ __forceinline__ void bwd_step(uint32_t code, T pold, T qold, const T 
     gs.Tx = fmaT(Dbr[r], sD[r], gs.Tx);
     Dbr[r] = Dbar;
     EMp[r] = EM;
-    Pb = fmaT(be[r], Qb[r], u);
+    Pb = fmaT(mu[r], Pb, v);  // one dependent operation per row on the carry chain
   }
   Ilast = Pb;
   Eprev = En;
