@@ -499,14 +499,15 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
     Xprev = Xn;
     if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
   }
-  const float inj = ANCH ? 0.f : C;
+  // the boundary row's injection, in lane 0 of each row only
+  const float inj = (ANCH || l != 0) ? 0.f : C;
   // One step at a time; the loops are unrolled by an even count so that the
   // loop-carried M/D/Q registers return to the same physical registers
   // without copies.
   auto main_step = [&](int tt) {
     const uint32_t code = cl[tt];
-    float Pin = dpp_shr1(inj, Plast);
-    float Xn = dpp_shr1(inj, Q[R - 1]);
+    float Pin = dpp_shr1_add(Plast, inj);
+    float Xn = dpp_shr1_add(Q[R - 1], inj);
     Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
     Xprev = Xn;
     if (ANCH) rmax = fmaxf(rmax, Q[R - 1]); else acc += Plast;
@@ -515,8 +516,8 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
     const uint32_t code = cl[tt];
     // past the haplotype end the boundary row injects nothing
     const float pinj = (!ANCH && tt - 1 > gp.m) ? 0.f : inj;
-    float Pin = dpp_shr1(pinj, Plast);
-    float Xn = dpp_shr1(inj, Q[R - 1]);
+    float Pin = dpp_shr1_add(Plast, pinj);
+    float Xn = dpp_shr1_add(Q[R - 1], inj);
     Plast = adiag_step<float, R, true>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
     Xprev = Xn;
     if (ANCH) {
@@ -793,6 +794,9 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
 
     float M[R], D[R], Q[R];
     float Plast = 0.f, Xprev = 0.f, acc = 0.f, rmax = 0.f, latch = 0.f;
+    // the injection as the DPP moves' old value: a lane-0 register for it
+    // (the pair-unit sweep's form) would push these instances past their
+    // register budgets
     const float inj = ANCH ? 0.f : C;
     auto main_step = [&](int tt) {
       const uint32_t code = cl[tt];
