@@ -111,6 +111,9 @@ __device__ __forceinline__ void injections(int t, int m, T C, T& pinj, T& qinj) 
 // The adjoint's per-lane sums (see the top of this file).
 template <typename T>
 struct GradSums {
+  // U, UL: sums of (mu Pb + be Qb) I over the interior rows and the read's
+  // last row; V, VL: their be Qb I parts; Tg, Tl, Tx: the gamma and
+  // extension terms
   T U, V, UL, VL, Tg, Tl, Tx;
 };
 
@@ -153,13 +156,11 @@ __device__ __forceinline__ void bwd_step(uint32_t code, T pold, T qold, const T 
     const T Mbar = Pb + Qb[r] + Dbr[r];
     const T EM = e[r] * Mbar;
     Er[r] = fmaT(Mbar * sQ[r], sw[r], Er[r]);
-    const T u = mu[r] * Pb, v = be[r] * Qb[r];
+    const T v = be[r] * Qb[r];
     if (r == R - 1) {
-      gs.UL = fmaT(u, Iv[r], gs.UL);
       gs.VL = fmaT(v, Iv[r], gs.VL);
       gs.Tl = fmaT(Qb[r], sD[r], gs.Tl);
     } else {
-      gs.U = fmaT(u, Iv[r], gs.U);
       gs.V = fmaT(v, Iv[r], gs.V);
       gs.Tg = fmaT(Qb[r], sD[r], gs.Tg);
     }
@@ -167,6 +168,10 @@ __device__ __forceinline__ void bwd_step(uint32_t code, T pold, T qold, const T 
     Dbr[r] = Dbar;
     EMp[r] = EM;
     Pb = fmaT(mu[r], Pb, v);  // one dependent operation per row on the carry chain
+    // the new carry is mu Pb + be Qb: U (UL) sums it times Iv, so the mu
+    // part alone is U - V, with no product of its own
+    if (r == R - 1) gs.UL = fmaT(Pb, Iv[r], gs.UL);
+    else gs.U = fmaT(Pb, Iv[r], gs.U);
   }
   Ilast = Pb;
   Eprev = En;
@@ -514,9 +519,10 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
         atomicAdd(&lds_de[q], (double)Er[r] * (sizeof(T) == 4 ? gt->dgood_de32[q] : gt->dgood_de[q]) * -1.0);
       }
     }
-    const double u = (double)gs.U + (last_lane ? 0.0 : (double)gs.UL);
+    const double uv = (double)gs.U + (last_lane ? 0.0 : (double)gs.UL);
     const double v = (double)gs.V + (last_lane ? 0.0 : (double)gs.VL);
-    const double w = last_lane ? (double)gs.UL + (double)gs.VL : 0.0;
+    const double u = uv - v;
+    const double w = last_lane ? (double)gs.UL : 0.0;
     const double tg = (double)gs.Tg + (last_lane ? 0.0 : (double)gs.Tl);
     const double tl = last_lane ? (double)gs.Tl : 0.0;
     const double mocc = last_lane ? (double)gp.n * wpair : 0.0;  // n w, once per pair
@@ -1038,7 +1044,7 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
     __threadfence();
     __syncthreads();
     GradSums<double> gs{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double Ufin = 0.0, Vfin = 0.0, Tfin = 0.0;  // the read's last row: lane 15, register R-1 of the last tile
+    double Ufin = 0.0, Tfin = 0.0;  // the read's last row: lane 15, register R-1 of the last tile
     for (int tile = ntiles - 1; tile >= 0; --tile) {
       const bool first = tile == 0, last = tile == ntiles - 1;
       const double gam_last = (l == GROUP - 1 && last) ? (ANCH ? tab->gap : 0.0) : gam;
@@ -1065,7 +1071,6 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
       // in every tile but the last, lane 15's last register row is an interior row
       if (last && l == GROUP - 1) {
         Ufin = gs.UL;
-        Vfin = gs.VL;
         Tfin = gs.Tl;
       } else {
         gs.U += gs.UL;
@@ -1084,11 +1089,11 @@ __global__ __launch_bounds__(64) void k_grad64t(BatchDev b, PlanDev p, const QTa
     if (gp.valid && !flag) {
       const double gg = (double)tab->gap, x = (double)tab->ext, a = 1.0 - 2.0 * gg;
       const bool last_lane = l == GROUP - 1;
-      const double ww = Ufin + Vfin;  // zero off lane 15
+      const double ww = Ufin;  // the last row's (mu Pb + be Qb) I sum; zero off lane 15
       const double mocc = last_lane ? (double)gp.n * wpair : 0.0;
-      acc[0] += -2.0 / a * (mocc - (gs.U + gs.V + ww)) + gs.V / (gg * a) + ww / gg + gs.Tg * gt->dgam_dg +
+      acc[0] += -2.0 / a * (mocc - (gs.U + ww)) + gs.V / (gg * a) + ww / gg + gs.Tg * gt->dgam_dg +
                 (ANCH ? Tfin : 0.0);
-      acc[1] += gs.U / x - gs.V / (1.0 - x) + gs.Tg * gt->dgam_dx + gs.Tx;
+      acc[1] += (gs.U - gs.V) / x - gs.V / (1.0 - x) + gs.Tg * gt->dgam_dx + gs.Tx;
     }
     __syncthreads();  // codes reused by the next unit
   }
