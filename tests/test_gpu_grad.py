@@ -71,6 +71,23 @@ def test_grad_matches_autograd_small(mode):
                                                                       np.max(np.abs(de)), st["flagged"]))
 
 
+@pytest.mark.parametrize("gap,ext", [(1e-4, 1e-3), (0.001, 0.02), (0.05, 0.9)])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_grad_extreme_transitions(mode, gap, ext):
+    """Gap and extension at the corners of their range.  A small extension
+    makes the mu part of the insertion sums (mu = x/4 over the normaliser)
+    small against their beta part, which the kernel forms as a difference of
+    two sums; a large one makes it dominant."""
+    rng = np.random.default_rng(300 + mode)
+    pairs = rand_pairs(rng, 24)
+    err = perturbed_err(2)
+    w = rng.uniform(0.5, 2.0, len(pairs))
+    out, g = engine.grad(batch_of(pairs, gap, ext), mode=mode, err=err, weights=w)
+    v, dg, dx, de = torch_ref.value_and_grad(pairs, mode, gap, ext, err, w)
+    np.testing.assert_allclose(out, v, rtol=0, atol=TOL_MIXED)
+    check_grad(g, dg, dx, de)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_grad_region_vs_autograd(mode):
     """8 reads x 8 haplotypes of the golden 64x8 region (150 bp reads, 200-400 bp haplotypes)."""
