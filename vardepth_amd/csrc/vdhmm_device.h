@@ -61,6 +61,16 @@ __device__ __forceinline__ float dpp_shl1_add(float src, float inj) {
          inj;
 }
 
+// A haplotype code read from LDS, held in a full register.  Neighbouring
+// 16-bit codes are read as one wide LDS load; without the empty asm the
+// compiler compares their halves with SDWA word selects, which issue at a
+// fraction of the rate of plain compares.  The asm is not volatile, so equal
+// codes still merge (the adjoint's recompute and backward share them).
+__device__ __forceinline__ uint32_t code_reg(uint32_t c) {
+  asm("" : "+v"(c));
+  return c;
+}
+
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 

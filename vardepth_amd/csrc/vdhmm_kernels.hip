@@ -505,7 +505,8 @@ __device__ __forceinline__ void unit32(const BatchDev& b, const PlanDev& p, cons
   // loop-carried M/D/Q registers return to the same physical registers
   // without copies.
   auto main_step = [&](int tt) {
-    const uint32_t code = cl[tt];
+    // four steps' codes are one wide LDS read: keep each in a full register
+    const uint32_t code = U == 4 ? code_reg(cl[tt]) : (uint32_t)cl[tt];
     float Pin = dpp_shr1_add(Plast, inj);
     float Xn = dpp_shr1_add(Q[R - 1], inj);
     Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
