@@ -43,20 +43,13 @@ __device__ __forceinline__ double dpp_shl1(double old, double src) {
   return __builtin_bit_cast(double, r);
 }
 
-// The same hand-offs for a lane-dependent boundary value: the edge lane of
-// each 16-lane row reads zero (bound_ctrl) and every lane adds inj, which is
-// zero except in the edge lane.  The shift folds into the add (one
-// v_add_f32 with a DPP operand, where the forms above take a move of the
-// boundary value and a DPP move), and the sum is exact: the other lanes add
-// +0 to a value that is never -0.
+// row_shr:1 for a lane-dependent boundary value: lane 0 of each 16-lane row
+// reads zero (bound_ctrl) and every lane adds inj, which is zero except in
+// lane 0.  The shift folds into the add (one v_add_f32 with a DPP operand,
+// where dpp_shr1 takes a move of the boundary value and a DPP move), and the
+// sum is exact: the other lanes add +0 to a value that is never -0.
 __device__ __forceinline__ float dpp_shr1_add(float src, float inj) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, src), 0x111, 0xF, 0xF,
-                                                               true)) +
-         inj;
-}
-
-__device__ __forceinline__ float dpp_shl1_add(float src, float inj) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, src), 0x101, 0xF, 0xF,
                                                                true)) +
          inj;
 }
