@@ -799,9 +799,19 @@ __device__ __forceinline__ void unit32_trie(const BatchDev& b, const PlanDev& p,
     // (the pair-unit sweep's form) would push these instances past their
     // register budgets
     const float inj = ANCH ? 0.f : C;
+    // Without the suffix continuation the P hand-off writes into a register
+    // carried across steps: lane 0 of each row is never written by the
+    // shift, so it keeps inj and the per-step move of inj is gone
+    float Pcar = inj;
     auto main_step = [&](int tt) {
       const uint32_t code = cl[tt];
-      float Pin = dpp_shr1(inj, Plast);
+      float Pin;
+      if constexpr (SFX) {
+        Pin = dpp_shr1(inj, Plast);
+      } else {
+        Pcar = dpp_shr1(Pcar, Plast);
+        Pin = Pcar;
+      }
       float Xn = dpp_shr1(inj, Q[R - 1]);
       Plast = adiag_step<float, R, false>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code);
       Xprev = Xn;
