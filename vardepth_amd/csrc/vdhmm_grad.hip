@@ -384,7 +384,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           injections<T, ANCH>(t, gp.m, C, pinj, qinj);
           const T Pin = dpp_shr1(pinj, Pl);
           const T Xn = dpp_shr1(qinj, Qr[R - 1]);
-          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, code_reg(cl[t]), qx, dx, k == 0);
+          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, code_reg(cl[t]), qx, dx,
+                               k == 0);
           Xp = Xn;
         }
         T sQ[GK][R], sD[GK][R], sPin[GK];
@@ -395,8 +396,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           injections<T, ANCH>(t, gp.m, C, pinj, qinj);
           const T Pin = dpp_shr1(pinj, Pl);
           const T Xn = dpp_shr1(qinj, Qr[R - 1]);
-          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, code_reg(cl[t]), sQ[s], sD[s],
-                               sub == 0 && s == 0);
+          Pl = gstep<T, R, TL>(Mr, Dr, Qr, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xp, code_reg(cl[t]), sQ[s],
+                               sD[s], sub == 0 && s == 0);
           Xp = Xn;
           sPin[s] = Pin;
         }
@@ -405,8 +406,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           const int t = t0 + s;
           // the anchored terminal enters lane 15 at step m + 17 (past the main blocks)
           const T qold = (TL && ANCH && t - 17 == gp.m) ? tseed : T(0);
-          bwd_step<T, R, TL>(code_reg(cl[t]), Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam, gam_last, xx,
-                             EMp, Dbr, Er, gs, Ilast, Eprev);
+          bwd_step<T, R, TL>(code_reg(cl[t]), Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam,
+                             gam_last, xx, EMp, Dbr, Er, gs, Ilast, Eprev);
         }
       }
     };
@@ -487,7 +488,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
             injections<T, ANCH>(t, gp.m, C, pinj, qinj);
             const T Pin = dpp_shr1(pinj, Plast);
             const T Xn = dpp_shr1(qinj, Q[R - 1]);
-            Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code_reg(cl[t]), sQ[s], sD[s]);
+            Plast = gstep<T, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code_reg(cl[t]), sQ[s],
+                                sD[s]);
             Xprev = Xn;
             sPin[s] = Pin;
           }
@@ -497,8 +499,8 @@ __device__ __forceinline__ void unit_grad(const BatchDev& b, const PlanDev& p, c
           if (s < cnt) {
             const int t = t0 + s;
             const T qold = (ANCH && t - 17 == gp.m) ? tseed : T(0);
-            bwd_step<T, R, true>(code_reg(cl[t]), Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam, gam_last,
-                                 xx, EMp, Dbr, Er, gs, Ilast, Eprev);
+            bwd_step<T, R, true>(code_reg(cl[t]), Pseed, qold, sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam,
+                                 gam_last, xx, EMp, Dbr, Er, gs, Ilast, Eprev);
           }
         }
       }
@@ -843,7 +845,8 @@ __device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t*
           inj(t, pinj, qinj);
           const double Pin = dpp_shr1(pinj, Plast);
           const double Xn = dpp_shr1(qinj, Q[R - 1]);
-          Plast = gstep<double, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code_reg(cl[t]), sQ[s], sD[s]);
+          Plast = gstep<double, R>(M, D, Q, gd, bd, mu, be, rc, gam, gam_last, xx, Pin, Xprev, code_reg(cl[t]), sQ[s],
+                                   sD[s]);
           Xprev = Xn;
           sPin[s] = Pin;
         }
@@ -852,8 +855,8 @@ __device__ __forceinline__ void tile_grad64(const GroupPair& gp, const uint16_t*
       for (int s = GKT - 1; s >= 0; --s) {
         if (s < cnt) {
           const int t = t0 + s;
-          bwd_step<double, R, true>(code_reg(cl[t]), seed_p(t), seed_q(t), sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc, pass, gam,
-                                    gam_last, xx, EMp, Dbr, Er, gs, Ilast, Eprev);
+          bwd_step<double, R, true>(code_reg(cl[t]), seed_p(t), seed_q(t), sQ[s], sD[s], sPin[s], gd, bd, mu, be, rc,
+                                    pass, gam, gam_last, xx, EMp, Dbr, Er, gs, Ilast, Eprev);
           // lane 0: adjoints of this step's injection, Pin (column t - 1) and
           // the diagonal source Qin (column t - 2), times the injection's
           // factors for the tile above
